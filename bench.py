@@ -1,0 +1,173 @@
+"""Headline benchmark: cell-updates/s per SSP-RK3 step of the CCZ4 engine on one MI355X.
+
+Workload (one "step" = one full RK3 step of the hot path over the block): a single
+Brill-Lindquist puncture, M = 1 at the origin, one uniform 128^3 block over [-16, 16]^3
+with 4 ghost cells (dx = 0.25, dt = CFL 0.25 * dx = 0.0625), kappa1 = 0.02, eta = 2,
+KO sigma = 0.1, FP64, level-0 ghost semantics of the reference (outflow fill before the
+step, 1/r fall-off after every stage). Initial data is generated on the device; the state
+is resident in HBM before timing starts. Synthetic data (no checkpoint is loaded).
+
+Prints one JSON line. Also reports the stage kernel's roofline (algorithmic bytes per
+launch / its average launch time from HIP events on the engine stream) and a CPU baseline
+timed on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+N = 128
+HALF = 16.0
+DT = 0.25 * (2 * HALF / N)
+BYTES_PER_CELL_STEP = 1408  # 3 stages x (22-field read + write) + 2 x 22-field read of u^n, FP64
+
+
+def cpu_baseline():
+    threads = min(16, len(os.sched_getaffinity(0)))
+    env = dict(os.environ)
+    env.update(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--n", str(N), "--half-width", str(HALF)],
+                         capture_output=True, text=True, env=env, timeout=900, cwd=ROOT)
+    if out.returncode != 0:
+        return {"value": None, "unit": "cell-updates/s", "cores": threads, "kind": None,
+                "sample": "failed: " + out.stderr[-300:]}
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    res.pop("seconds", None)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the level would shard "
+              "per block across GPUs; see the sharding design). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import granite_amd as ga
+
+    eng = ga.Engine(0)
+    blk = ga.GridBlock(0, 0, (N, N, N), (-HALF,) * 3, (HALF,) * 3, 4)
+    lvl = ga.Level([blk], eng)
+    lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
+    p = ga.CCZ4Params(kappa1=0.02, kappa2=0.0, eta=2.0, ko_sigma=0.1)
+    step = dict(precision=ga.PREC_FP64, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
+
+    for _ in range(args.warmup):
+        lvl.rk3_step(p, DT, use_graph=True, **step)
+    lvl.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lvl.rk3_step(p, DT, use_graph=True, **step)
+    lvl.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    cells = N ** 3
+    value = cells / (ms_per_step * 1e-3)
+
+    # per-kernel device times (HIP events on the engine stream, eager launches)
+    lvl.profile(True)
+    for k in range(3):
+        lvl.profile_read(k)
+    for _ in range(args.profile_steps):
+        lvl.rk3_step(p, DT, use_graph=False, **step)
+    t_stage, n_stage = lvl.profile_read(0)
+    t_final, n_final = lvl.profile_read(1)
+    t_ghost, n_ghost = lvl.profile_read(2)
+    lvl.profile(False)
+    nonfinite = lvl.nonfinite()
+    sumsq, count = lvl.constraints_l2()
+
+    # stage kernels of RK3 stages 1 and 2 alternate: stage 1 reads u (stencil) and writes
+    # u1 = 352 B/cell; stage 2 reads u1 (stencil) and u^n and writes u2 = 528 B/cell
+    stage_avg_ms = t_stage / max(n_stage, 1)
+    stage_bytes = (352 + 528) / 2 * cells
+    final_avg_ms = t_final / max(n_final, 1)
+    final_bytes = 528 * cells
+    achieved = stage_bytes / (stage_avg_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "stage_kernel_traffic_bl128.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+
+    cpu = {"value": None, "unit": "cell-updates/s", "cores": 0, "kind": None, "sample": "skipped (--no-cpu)"}
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+
+    line = {
+        "metric": "cell-updates/s per RK3 step on 1 GPU, FP64 vs mixed/batched, at matched H-constraint",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "single Brill-Lindquist puncture M=1, one 128^3 block on [-16,16]^3, ng=4, FP64 RK3 step",
+            "block": [N, N, N],
+            "ghosts": 4,
+            "dx": 2 * HALF / N,
+            "dt": DT,
+            "precision": "fp64",
+            "stage_tile": list(_tile(ga)),
+            "graph": True,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "stage_kernel<MODE_STAGE> (RK3 stages 1-2: RHS + KO + combine + floors)",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": stage_bytes,
+            "avg_launch_ms": stage_avg_ms,
+            "launches": n_stage,
+        },
+        "cpu_baseline": cpu,
+        "step_algorithmic_GBs": BYTES_PER_CELL_STEP * value / 1e9,
+        "kernels_ms": {
+            "stage_1_2_avg": stage_avg_ms,
+            "stage_3_avg": final_avg_ms,
+            "stage_3_GBs": final_bytes / (final_avg_ms * 1e-3) / 1e9 if n_final else None,
+            "ghost_fill_avg": t_ghost / max(n_ghost, 1),
+        },
+        "checks": {"nonfinite": nonfinite, "ham_l2": (sumsq / count) ** 0.5 if count else None},
+    }
+    if cpu.get("value"):
+        line["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+    lvl.close()
+    eng.close()
+
+
+def _tile(ga):
+    import ctypes
+
+    t = (ctypes.c_int * 3)()
+    ga.library().gr_stage_tile(0, t)
+    return tuple(t)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,584 @@
+// C ABI of the CCZ4 RHS + SSP-RK3 engine (declared in include/granite_ccz4.h).
+//
+// Host pointers are borrowed for the duration of a call. Device memory is owned by the
+// engine (compatibility-mode staging buffers) and by levels (state + two stage buffers per
+// block). No exception crosses the boundary: every failure becomes a status code and a
+// message readable through gr_last_error().
+#include "../../include/granite_ccz4.h"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+struct gr_engine {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // compatibility-mode staging buffers (grown on demand, doubles)
+    double* d_u = nullptr;
+    double* d_r = nullptr;
+    size_t cap_state = 0;
+    size_t cap_r = 0;
+    double* d_src = nullptr;
+    size_t cap_src = 0;
+    double* d_aux = nullptr;
+    size_t cap_aux = 0;
+};
+
+struct gr_level {
+    gr_engine* eng = nullptr;
+    std::vector<gr::BlockDesc> blk;
+    std::vector<double*> u, s1, s2;
+    double* ham = nullptr;     // monitor scratch (max stride)
+    double* partial = nullptr; // reduction partials
+    double* red = nullptr;     // reduction result (1 double)
+    double* bh = nullptr;      // puncture masses/positions for initial data
+    int partial_cap = 1024;
+    // captured RK3 step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool have_graph = false;
+    double gkey[10] = {0};
+    // optional per-kernel event timing of eager steps: kind -> list of (start, stop)
+    bool profile = false;
+    std::vector<hipEvent_t> ev[3];
+    size_t ev_used[3] = {0, 0, 0};
+};
+
+namespace {
+
+constexpr const char* kVersion = "granite-ccz4-mi355x 0.1";
+
+int fail(gr_engine* e, int code, const std::string& msg) {
+    if (e) e->err = msg;
+    return code;
+}
+
+int hipfail(gr_engine* e, hipError_t st, const char* what) {
+    if (e) e->err = std::string(what) + ": " + hipGetErrorString(st);
+    return (int)st;
+}
+
+#define GR_HIP(e, call)                                                                                            \
+    do {                                                                                                           \
+        hipError_t st_ = (call);                                                                                   \
+        if (st_ != hipSuccess) return hipfail((e), st_, #call);                                                    \
+    } while (0)
+
+int check_geom(gr_engine* e, const gr_geom* g, gr::BlockDesc* out) {
+    if (!g) return fail(e, GR_ERR_ARG, "null geometry");
+    for (int d = 0; d < 3; ++d) {
+        if (g->n[d] < 1) return fail(e, GR_ERR_GEOM, "n must be >= 1 on every axis");
+        if (!(g->dx[d] > 0.0) || !std::isfinite(g->dx[d])) return fail(e, GR_ERR_GEOM, "dx must be positive");
+    }
+    // the one-sided advective stencil reaches 4 cells (src/spacetime/ccz4.cpp:107-152)
+    if (g->ng < 4) return fail(e, GR_ERR_GEOM, "ng must be >= 4 (the upwind stencil reaches 4 cells)");
+    const long nt0 = g->n[0] + 2L * g->ng, nt1 = g->n[1] + 2L * g->ng, nt2 = g->n[2] + 2L * g->ng;
+    const long stride = nt0 * nt1 * nt2;
+    if (stride * 22 >= (1L << 31) * 8L) return fail(e, GR_ERR_GEOM, "block too large");
+    for (int d = 0; d < 3; ++d) {
+        out->n[d] = g->n[d];
+        out->nt[d] = g->n[d] + 2 * g->ng;
+        out->lo[d] = g->lo[d];
+        out->dx[d] = g->dx[d];
+    }
+    out->ng = g->ng;
+    out->stride = stride;
+    return GR_OK;
+}
+
+int check_params(gr_engine* e, const gr_ccz4_params* p) {
+    if (!p) return fail(e, GR_ERR_ARG, "null parameters");
+    if (!(p->chi_blend_width != 0.0)) return fail(e, GR_ERR_ARG, "chi_blend_width must be non-zero");
+    return GR_OK;
+}
+
+gr::CellParams cell_params(const gr_ccz4_params* p) {
+    return gr::CellParams{p->kappa1, p->kappa2, p->eta, p->chi_blend_center, p->chi_blend_width};
+}
+
+int grow(gr_engine* e, double** buf, size_t* cap, size_t n) {
+    if (*cap >= n) return GR_OK;
+    if (*buf) GR_HIP(e, hipFree(*buf));
+    *buf = nullptr;
+    *cap = 0;
+    GR_HIP(e, hipMalloc((void**)buf, n * sizeof(double)));
+    *cap = n;
+    return GR_OK;
+}
+
+int use_device(gr_engine* e) {
+    if (!e) return GR_ERR_ARG;
+    GR_HIP(e, hipSetDevice(e->device));
+    return GR_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* gr_version(void) { return kVersion; }
+
+int gr_stage_tile(int precision, int* t3) {
+    if (!t3) return GR_ERR_ARG;
+    if (precision != GR_PREC_FP64) return GR_ERR_ARG;
+    gr::stage_tile_fp64(t3);
+    return GR_OK;
+}
+
+int gr_engine_create(int device, gr_engine** out) {
+    if (!out) return GR_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return GR_ERR_NODEV;
+    if (device < 0 || device >= count) return GR_ERR_NODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return GR_ERR_NODEV;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return GR_ERR_NODEV;
+    gr_engine* e = new gr_engine();
+    e->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess) {
+        delete e;
+        return GR_ERR_NODEV;
+    }
+    *out = e;
+    return GR_OK;
+}
+
+int gr_engine_destroy(gr_engine* e) {
+    if (!e) return GR_ERR_ARG;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->d_u) (void)hipFree(e->d_u);
+    if (e->d_r) (void)hipFree(e->d_r);
+    if (e->d_src) (void)hipFree(e->d_src);
+    if (e->d_aux) (void)hipFree(e->d_aux);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return GR_OK;
+}
+
+const char* gr_last_error(const gr_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int gr_engine_sync(gr_engine* e) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    return GR_OK;
+}
+
+int gr_ccz4_rhs(gr_engine* e, const gr_geom* geom, const gr_ccz4_params* prm, int nvars, const double* u,
+                double* rhs, const double* rho, const double* Si, const double* Sij, const double* S) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    gr::BlockDesc b;
+    if ((rc = check_geom(e, geom, &b))) return rc;
+    if ((rc = check_params(e, prm))) return rc;
+    if (nvars != GR_NUM_VARS) return fail(e, GR_ERR_NVARS, "nvars must be 22");
+    if (!u || !rhs) return fail(e, GR_ERR_ARG, "null state or rhs");
+    const int nsrc = (rho != nullptr) + (Si != nullptr) + (Sij != nullptr) + (S != nullptr);
+    if (nsrc != 0 && nsrc != 4) return fail(e, GR_ERR_ARG, "matter sources must be all null or all non-null");
+    const size_t n = 22 * (size_t)b.stride;
+    if ((rc = grow(e, &e->d_u, &e->cap_state, n))) return rc;
+    if ((rc = grow(e, &e->d_r, &e->cap_r, n))) return rc;
+    gr::Sources src{nullptr, nullptr, nullptr, nullptr};
+    if (nsrc == 4) {
+        if ((rc = grow(e, &e->d_src, &e->cap_src, 11 * (size_t)b.stride))) return rc;
+        double* d = e->d_src;
+        GR_HIP(e, hipMemcpyAsync(d, rho, b.stride * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        GR_HIP(e, hipMemcpyAsync(d + b.stride, Si, 3 * b.stride * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        GR_HIP(e, hipMemcpyAsync(d + 4 * b.stride, Sij, 6 * b.stride * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        GR_HIP(e, hipMemcpyAsync(d + 10 * b.stride, S, b.stride * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        src = gr::Sources{d, d + b.stride, d + 4 * b.stride, d + 10 * b.stride};
+    }
+    GR_HIP(e, hipMemcpyAsync(e->d_u, u, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    GR_HIP(e, hipMemcpyAsync(e->d_r, rhs, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    GR_HIP(e, gr::launch_stage_fp64(b, e->d_u, e->d_u, e->d_r, src, cell_params(prm), prm->ko_sigma, gr::MODE_RHS,
+                                    gr::StageCoef{0.0, 0.0, 0.0}, e->stream));
+    GR_HIP(e, hipMemcpyAsync(rhs, e->d_r, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    return GR_OK;
+}
+
+int gr_ccz4_rhs_device(gr_engine* e, const gr_geom* geom, const gr_ccz4_params* prm, const double* d_u,
+                       double* d_rhs, const double* d_rho, const double* d_Si, const double* d_Sij,
+                       const double* d_S) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    gr::BlockDesc b;
+    if ((rc = check_geom(e, geom, &b))) return rc;
+    if ((rc = check_params(e, prm))) return rc;
+    if (!d_u || !d_rhs) return fail(e, GR_ERR_ARG, "null state or rhs");
+    const int nsrc = (d_rho != nullptr) + (d_Si != nullptr) + (d_Sij != nullptr) + (d_S != nullptr);
+    if (nsrc != 0 && nsrc != 4) return fail(e, GR_ERR_ARG, "matter sources must be all null or all non-null");
+    gr::Sources src{d_rho, d_Si, d_Sij, d_S};
+    GR_HIP(e, gr::launch_stage_fp64(b, d_u, d_u, d_rhs, src, cell_params(prm), prm->ko_sigma, gr::MODE_RHS,
+                                    gr::StageCoef{0.0, 0.0, 0.0}, e->stream));
+    return GR_OK;
+}
+
+int gr_ccz4_dissipation(gr_engine* e, const gr_geom* geom, const gr_ccz4_params* prm, int nvars, const double* u,
+                        double* rhs) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    gr::BlockDesc b;
+    if ((rc = check_geom(e, geom, &b))) return rc;
+    if (!prm) return fail(e, GR_ERR_ARG, "null parameters");
+    if (nvars != GR_NUM_VARS) return fail(e, GR_ERR_NVARS, "nvars must be 22");
+    if (!u || !rhs) return fail(e, GR_ERR_ARG, "null state or rhs");
+    if (!(prm->ko_sigma > 0.0)) return GR_OK; // src/spacetime/ccz4.cpp:873-874
+    const size_t n = 22 * (size_t)b.stride;
+    if ((rc = grow(e, &e->d_aux, &e->cap_aux, 2 * n))) return rc;
+    double* du = e->d_aux;
+    double* dr = e->d_aux + n;
+    GR_HIP(e, hipMemcpyAsync(du, u, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    GR_HIP(e, hipMemcpyAsync(dr, rhs, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    GR_HIP(e, gr::launch_dissipation(b, du, dr, prm->ko_sigma, e->stream));
+    GR_HIP(e, hipMemcpyAsync(rhs, dr, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    return GR_OK;
+}
+
+int gr_ccz4_constraints(gr_engine* e, const gr_geom* geom, const double* u, double* ham, double* mom) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    gr::BlockDesc b;
+    if ((rc = check_geom(e, geom, &b))) return rc;
+    if (!u || !ham || !mom) return fail(e, GR_ERR_ARG, "null state, ham or mom");
+    const size_t n = 22 * (size_t)b.stride;
+    if ((rc = grow(e, &e->d_aux, &e->cap_aux, n + 4 * (size_t)b.stride))) return rc;
+    double* du = e->d_aux;
+    double* dh = e->d_aux + n;
+    double* dm = dh + b.stride;
+    GR_HIP(e, hipMemcpyAsync(du, u, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    GR_HIP(e, gr::launch_constraints(b, du, dh, dm, e->stream));
+    GR_HIP(e, hipMemcpyAsync(ham, dh, b.stride * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    GR_HIP(e, hipMemcpyAsync(mom, dm, 3 * b.stride * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    return GR_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// levels
+// ---------------------------------------------------------------------------------------
+
+int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** out) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!out || !geoms || nblocks < 1) return fail(e, GR_ERR_ARG, "bad level arguments");
+    *out = nullptr;
+    gr_level* L = new gr_level();
+    L->eng = e;
+    long maxstride = 0;
+    for (int i = 0; i < nblocks; ++i) {
+        gr::BlockDesc b;
+        if ((rc = check_geom(e, &geoms[i], &b))) {
+            gr_level_destroy(L);
+            return rc;
+        }
+        L->blk.push_back(b);
+        if (b.stride > maxstride) maxstride = b.stride;
+        double *u = nullptr, *a = nullptr, *c = nullptr;
+        const size_t bytes = 22 * (size_t)b.stride * sizeof(double);
+        hipError_t st = hipMalloc((void**)&u, bytes);
+        if (st == hipSuccess) st = hipMalloc((void**)&a, bytes);
+        if (st == hipSuccess) st = hipMalloc((void**)&c, bytes);
+        L->u.push_back(u);
+        L->s1.push_back(a);
+        L->s2.push_back(c);
+        if (st != hipSuccess) {
+            gr_level_destroy(L);
+            return hipfail(e, st, "hipMalloc(level block)");
+        }
+    }
+    hipError_t st = hipMalloc((void**)&L->ham, maxstride * sizeof(double));
+    if (st == hipSuccess) st = hipMalloc((void**)&L->partial, L->partial_cap * sizeof(double));
+    if (st == hipSuccess) st = hipMalloc((void**)&L->red, sizeof(double));
+    if (st != hipSuccess) {
+        gr_level_destroy(L);
+        return hipfail(e, st, "hipMalloc(level scratch)");
+    }
+    *out = L;
+    return GR_OK;
+}
+
+int gr_level_destroy(gr_level* L) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    if (e) {
+        (void)hipSetDevice(e->device);
+        (void)hipStreamSynchronize(e->stream);
+    }
+    if (L->exec) (void)hipGraphExecDestroy(L->exec);
+    if (L->graph) (void)hipGraphDestroy(L->graph);
+    for (auto& v : L->ev)
+        for (hipEvent_t ev : v) (void)hipEventDestroy(ev);
+    for (double* p : L->u)
+        if (p) (void)hipFree(p);
+    for (double* p : L->s1)
+        if (p) (void)hipFree(p);
+    for (double* p : L->s2)
+        if (p) (void)hipFree(p);
+    if (L->ham) (void)hipFree(L->ham);
+    if (L->partial) (void)hipFree(L->partial);
+    if (L->red) (void)hipFree(L->red);
+    if (L->bh) (void)hipFree(L->bh);
+    delete L;
+    return GR_OK;
+}
+
+static int level_block(gr_level* L, int b) {
+    if (!L) return GR_ERR_ARG;
+    if (b < 0 || b >= (int)L->blk.size()) return fail(L->eng, GR_ERR_ARG, "block index out of range");
+    return use_device(L->eng);
+}
+
+int gr_level_upload(gr_level* L, int b, const double* host_u) {
+    int rc = level_block(L, b);
+    if (rc) return rc;
+    if (!host_u) return fail(L->eng, GR_ERR_ARG, "null host state");
+    gr_engine* e = L->eng;
+    GR_HIP(e, hipMemcpyAsync(L->u[b], host_u, 22 * L->blk[b].stride * sizeof(double), hipMemcpyHostToDevice,
+                             e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    return GR_OK;
+}
+
+int gr_level_download(gr_level* L, int b, double* host_u) {
+    int rc = level_block(L, b);
+    if (rc) return rc;
+    if (!host_u) return fail(L->eng, GR_ERR_ARG, "null host state");
+    gr_engine* e = L->eng;
+    GR_HIP(e, hipMemcpyAsync(host_u, L->u[b], 22 * L->blk[b].stride * sizeof(double), hipMemcpyDeviceToHost,
+                             e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    return GR_OK;
+}
+
+int gr_level_state(gr_level* L, int b, double** d_u) {
+    int rc = level_block(L, b);
+    if (rc) return rc;
+    if (!d_u) return fail(L->eng, GR_ERR_ARG, "null output");
+    *d_u = L->u[b];
+    return GR_OK;
+}
+
+int gr_level_init_brill_lindquist(gr_level* L, int nbh, const double* mass, const double* pos) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (nbh < 0 || nbh > 64 || (nbh > 0 && (!mass || !pos))) return fail(e, GR_ERR_ARG, "bad puncture list");
+    if (!L->bh) GR_HIP(e, hipMalloc((void**)&L->bh, 4 * 64 * sizeof(double)));
+    std::vector<double> h(4 * 64, 0.0);
+    for (int i = 0; i < nbh; ++i) {
+        h[i] = mass[i];
+        for (int d = 0; d < 3; ++d) h[64 + 3 * i + d] = pos[3 * i + d];
+    }
+    GR_HIP(e, hipMemcpyAsync(L->bh, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    for (size_t b = 0; b < L->blk.size(); ++b)
+        GR_HIP(e, gr::launch_init_bl(L->blk[b], L->u[b], nbh, L->bh, L->bh + 64, e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    return GR_OK;
+}
+
+int gr_level_add_noise(gr_level* L, uint64_t seed, double amp, uint32_t mask) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    for (size_t b = 0; b < L->blk.size(); ++b)
+        GR_HIP(e, gr::launch_add_noise(L->blk[b], L->u[b], seed, amp, mask, e->stream));
+    return GR_OK;
+}
+
+int gr_level_fill_ghosts(gr_level* L, int mode) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (mode < 0 || mode > 2) return fail(e, GR_ERR_ARG, "ghost mode must be 0, 1 or 2");
+    for (size_t b = 0; b < L->blk.size(); ++b) GR_HIP(e, gr::launch_fill_ghosts(L->blk[b], L->u[b], mode, e->stream));
+    return GR_OK;
+}
+
+} // extern "C"
+
+// records a start/stop event pair around `launch` when profiling is on
+template <class F>
+static hipError_t timed(gr_level* L, int kind, F&& launch) {
+    if (!L->profile) return launch();
+    auto& v = L->ev[kind];
+    size_t& used = L->ev_used[kind];
+    while (v.size() < used + 2) {
+        hipEvent_t ev;
+        hipError_t st = hipEventCreate(&ev);
+        if (st != hipSuccess) return st;
+        v.push_back(ev);
+    }
+    hipError_t st = hipEventRecord(v[used], L->eng->stream);
+    if (st != hipSuccess) return st;
+    if ((st = launch()) != hipSuccess) return st;
+    if ((st = hipEventRecord(v[used + 1], L->eng->stream)) != hipSuccess) return st;
+    used += 2;
+    return hipSuccess;
+}
+
+static hipError_t enqueue_rk3(gr_level* L, const gr_ccz4_params* prm, double dt, int bc, int pre) {
+    gr_engine* e = L->eng;
+    const gr::CellParams P = cell_params(prm);
+    const gr::Sources none{nullptr, nullptr, nullptr, nullptr};
+    // stage weights of TimeIntegrator::sspRK3Step, src/main.cpp:673,679,685
+    const gr::StageCoef c1{1.0, 0.0, 1.0 * dt};
+    const gr::StageCoef c2{0.75, 0.25, 0.25 * dt};
+    const gr::StageCoef c3{1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt};
+    for (size_t b = 0; b < L->blk.size(); ++b) {
+        const gr::BlockDesc& B = L->blk[b];
+        double *u = L->u[b], *s1 = L->s1[b], *s2 = L->s2[b];
+        hipStream_t s = e->stream;
+        hipError_t st;
+        if (pre >= 0 && (st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, u, pre, s); })) != hipSuccess)
+            return st;
+        if ((st = timed(L, 0, [&] {
+                 return gr::launch_stage_fp64(B, u, u, s1, none, P, prm->ko_sigma, gr::MODE_STAGE, c1, s);
+             })) != hipSuccess)
+            return st;
+        if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, s1, bc, s); })) != hipSuccess) return st;
+        if ((st = timed(L, 0, [&] {
+                 return gr::launch_stage_fp64(B, s1, u, s2, none, P, prm->ko_sigma, gr::MODE_STAGE, c2, s);
+             })) != hipSuccess)
+            return st;
+        if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, s2, bc, s); })) != hipSuccess) return st;
+        if ((st = timed(L, 1, [&] {
+                 return gr::launch_stage_fp64(B, s2, u, u, none, P, prm->ko_sigma, gr::MODE_FINAL, c3, s);
+             })) != hipSuccess)
+            return st;
+        if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, u, bc, s); })) != hipSuccess) return st;
+    }
+    return hipSuccess;
+}
+
+extern "C" {
+
+int gr_level_rk3_step(gr_level* L, const gr_ccz4_params* prm, double dt, int precision, int bc_mode, int pre_fill,
+                      int use_graph) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if ((rc = check_params(e, prm))) return rc;
+    if (!std::isfinite(dt)) return fail(e, GR_ERR_ARG, "dt must be finite");
+    if (precision != GR_PREC_FP64) return fail(e, GR_ERR_ARG, "this build implements the FP64 stage kernels only");
+    if (bc_mode < 0 || bc_mode > 2) return fail(e, GR_ERR_ARG, "bc_mode must be 0, 1 or 2");
+    if (pre_fill < -1 || pre_fill > 2) return fail(e, GR_ERR_ARG, "pre_fill must be -1, 0, 1 or 2");
+    if (!use_graph || L->profile) {
+        GR_HIP(e, enqueue_rk3(L, prm, dt, bc_mode, pre_fill));
+        return GR_OK;
+    }
+    const double key[10] = {dt, prm->kappa1, prm->kappa2, prm->eta, prm->ko_sigma, prm->chi_blend_center,
+                            prm->chi_blend_width, (double)precision, (double)bc_mode, (double)pre_fill};
+    if (!L->have_graph || std::memcmp(key, L->gkey, sizeof(key)) != 0) {
+        if (L->exec) GR_HIP(e, hipGraphExecDestroy(L->exec));
+        if (L->graph) GR_HIP(e, hipGraphDestroy(L->graph));
+        L->exec = nullptr;
+        L->graph = nullptr;
+        L->have_graph = false;
+        GR_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t st = enqueue_rk3(L, prm, dt, bc_mode, pre_fill);
+        hipGraph_t g = nullptr;
+        hipError_t st2 = hipStreamEndCapture(e->stream, &g);
+        if (st != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hipfail(e, st, "capture of the RK3 step");
+        }
+        if (st2 != hipSuccess) return hipfail(e, st2, "hipStreamEndCapture");
+        L->graph = g;
+        GR_HIP(e, hipGraphInstantiate(&L->exec, L->graph, nullptr, nullptr, 0));
+        std::memcpy(L->gkey, key, sizeof(key));
+        L->have_graph = true;
+    }
+    GR_HIP(e, hipGraphLaunch(L->exec, e->stream));
+    return GR_OK;
+}
+
+int gr_level_profile(gr_level* L, int enable) {
+    if (!L) return GR_ERR_ARG;
+    L->profile = enable != 0;
+    return GR_OK;
+}
+
+int gr_level_profile_read(gr_level* L, int kind, double* total_ms, int* launches) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (kind < 0 || kind > 2 || !total_ms || !launches) return fail(e, GR_ERR_ARG, "bad profile query");
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    double t = 0.0;
+    for (size_t i = 0; i + 1 < L->ev_used[kind]; i += 2) {
+        float ms = 0.f;
+        GR_HIP(e, hipEventElapsedTime(&ms, L->ev[kind][i], L->ev[kind][i + 1]));
+        t += ms;
+    }
+    *total_ms = t;
+    *launches = (int)(L->ev_used[kind] / 2);
+    L->ev_used[kind] = 0;
+    return GR_OK;
+}
+
+int gr_level_sync(gr_level* L) {
+    if (!L) return GR_ERR_ARG;
+    return gr_engine_sync(L->eng);
+}
+
+int gr_level_constraints_l2(gr_level* L, double* sumsq, long long* count) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!sumsq || !count) return fail(e, GR_ERR_ARG, "null output");
+    double total = 0.0;
+    long long n = 0;
+    for (size_t b = 0; b < L->blk.size(); ++b) {
+        const gr::BlockDesc& B = L->blk[b];
+        int nparts = 0;
+        GR_HIP(e, gr::launch_constraints(B, L->u[b], L->ham, nullptr, e->stream));
+        GR_HIP(e, gr::launch_sumsq_interior(B, L->ham, L->partial, L->partial_cap, &nparts, e->stream));
+        GR_HIP(e, gr::launch_sum_partials(L->partial, nparts, L->red, e->stream));
+        double h = 0.0;
+        GR_HIP(e, hipMemcpyAsync(&h, L->red, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        GR_HIP(e, hipStreamSynchronize(e->stream));
+        total += h;
+        n += (long long)B.n[0] * B.n[1] * B.n[2];
+    }
+    *sumsq = total;
+    *count = n;
+    return GR_OK;
+}
+
+int gr_level_nonfinite(gr_level* L, long long* count) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!count) return fail(e, GR_ERR_ARG, "null output");
+    double total = 0.0;
+    for (size_t b = 0; b < L->blk.size(); ++b) {
+        int nparts = 0;
+        GR_HIP(e, gr::launch_count_nonfinite(L->blk[b], L->u[b], L->partial, L->partial_cap, &nparts, e->stream));
+        GR_HIP(e, gr::launch_sum_partials(L->partial, nparts, L->red, e->stream));
+        double h = 0.0;
+        GR_HIP(e, hipMemcpyAsync(&h, L->red, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        GR_HIP(e, hipStreamSynchronize(e->stream));
+        total += h;
+    }
+    *count = (long long)total;
+    return GR_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,786 @@
+// gfx950 kernels of the CCZ4 RHS + SSP-RK3 stage engine (FP64 form).
+//
+// Stage kernel layout: one workgroup per tile of TX x TY x TZ interior cells, one thread
+// per cell. The 22 fields are streamed one at a time through an LDS box holding the tile
+// plus the stencil halo (4 cells along each axis, 1 cell on the edges used by the mixed
+// second derivatives); each field is read from HBM once per tile in coalesced rows along x.
+// The per-cell arithmetic is ccz4_cell.hpp; the RK3 combine, the floors and (last stage)
+// the algebraic constraints are applied to each value before it is stored, so one launch
+// per stage reads the stage input and u^n and writes the new stage state.
+#include "kernels.hpp"
+
+#include <cstdint>
+
+namespace gr {
+
+namespace {
+
+constexpr int HALO = 4;
+
+// ------------------------------------------------------------------------------------
+// exact (contraction-free) pointwise pieces shared with the reference
+// ------------------------------------------------------------------------------------
+
+// d = a1*s1 + a2*s2 + a3*dt*r, src/main.cpp:328-329
+__device__ __forceinline__ double combine_exact(double a1, double s1, double a2, double s2, double a3dt, double r) {
+#pragma clang fp contract(off)
+    return a1 * s1 + a2 * s2 + a3dt * r;
+}
+
+__device__ __forceinline__ double floor_chi(double c) {
+    // src/main.cpp:598-602
+    if (!__builtin_isfinite(c) || c < 1.0e-4) c = 1.0e-4;
+    if (c > 1.5) c = 1.5;
+    return c;
+}
+
+__device__ __forceinline__ double floor_alpha(double a) {
+    // src/main.cpp:603-605
+    if (!__builtin_isfinite(a) || a < 1.0e-6) a = 1.0e-6;
+    return a;
+}
+
+// det(gt) -> 1 by a cube-root rescale (or reset to identity), then remove the trace of A~
+// with the cofactors of the rescaled metric, src/main.cpp:622-668
+__device__ __forceinline__ void algebraic_exact(double g[6], double A[6]) {
+#pragma clang fp contract(off)
+    double &xx = g[0], &xy = g[1], &xz = g[2], &yy = g[3], &yz = g[4], &zz = g[5];
+    const double det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+    if (__builtin_isfinite(det) && det > 1.0e-10) {
+        const double s = cbrt(1.0 / det);
+        for (int a = 0; a < 6; ++a) g[a] *= s;
+    } else {
+        xx = 1.0; xy = 0.0; xz = 0.0; yy = 1.0; yz = 0.0; zz = 1.0;
+    }
+    const double cxx = yy * zz - yz * yz;
+    const double cxy = xz * yz - xy * zz;
+    const double cxz = xy * yz - xz * yy;
+    const double cyy = xx * zz - xz * xz;
+    const double cyz = xz * xy - xx * yz;
+    const double czz = xx * yy - xy * xy;
+    const double tr = cxx * A[0] + cyy * A[3] + czz * A[5] + 2.0 * (cxy * A[1] + cxz * A[2] + cyz * A[4]);
+    const double t3 = tr / 3.0;
+    for (int a = 0; a < 6; ++a) A[a] -= t3 * g[a];
+}
+
+// ------------------------------------------------------------------------------------
+// LDS staging of one field of a tile
+// ------------------------------------------------------------------------------------
+template <int TX, int TY, int TZ>
+struct TileGeom {
+    static constexpr int NT = TX * TY * TZ;
+    static constexpr int BX = TX + 2 * HALO, BY = TY + 2 * HALO, BZ = TZ + 2 * HALO;
+    static constexpr int NLDS = BX * BY * BZ;
+    // sub-boxes of the halo region that some stencil reads (x0,x1,y0,y1,z0,z1)
+    static constexpr int NBOX = 7;
+    static constexpr int box[NBOX][6] = {
+        {-HALO, TX + HALO, 0, TY, 0, TZ},          // tile + x halo
+        {-1, TX + 1, -HALO, 0, 0, TZ},             // y halo (low) with the xy edge column
+        {-1, TX + 1, TY, TY + HALO, 0, TZ},        // y halo (high)
+        {-1, TX + 1, -1, TY + 1, -1, 0},           // z = -1 layer with xz / yz edges
+        {-1, TX + 1, -1, TY + 1, TZ, TZ + 1},      // z = TZ layer
+        {0, TX, 0, TY, -HALO, -1},                 // z halo (low), axis taps only
+        {0, TX, 0, TY, TZ + 1, TZ + HALO},         // z halo (high)
+    };
+    static constexpr int count(int b) {
+        return (box[b][1] - box[b][0]) * (box[b][3] - box[b][2]) * (box[b][5] - box[b][4]);
+    }
+    static constexpr int start(int b) { return b == 0 ? 0 : start(b - 1) + count(b - 1); }
+    static constexpr int TOTAL = start(NBOX - 1) + count(NBOX - 1);
+    __host__ __device__ static constexpr int lds(int x, int y, int z) {
+        return ((z + HALO) * BY + (y + HALO)) * BX + (x + HALO);
+    }
+};
+
+// Per-thread input/output of one cell of a tile: stages the fields into LDS for the
+// stencils (Stager interface of ccz4_cell) and combines/stores the results (Sink interface).
+//
+// Every thread owns the same NL entries of the tile's halo box for all fields, so the
+// LDS slot and the global offset of each entry are computed once. Field loads are issued
+// PF fields ahead of their use (in the fixed FIELD_ORDER), so that their HBM latency
+// overlaps the evaluation of the fields in between; the pointwise u^n value needed by the
+// RK3 combine of a field travels with that field's prefetch.
+template <int TX, int TY, int TZ, int MODE, int PF>
+struct TileIO {
+    using T = TileGeom<TX, TY, TZ>;
+    static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
+
+    double* lds;
+    const double* in;
+    const double* un;
+    double* out;
+    long stride;
+    long cidx;
+    int cl;
+    bool active;
+    bool ko_on;
+    StageCoef cf;
+    int loff[NL]; // LDS slot of each owned entry (T::NLDS = discard slot)
+    int goff[NL]; // offset of the entry inside one field of the block
+    double buf[PF][NL];
+    double unb[PF];
+    double unv[NV];
+    double hg[6], hA[6]; // last stage: new metric and A~ held for the algebraic constraints
+
+    __device__ void init(int tid, int ox, int oy, int oz, int nt0, int nt1, int nt2) {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const int p = tid + i * T::NT;
+            int b = 0, q = p;
+#pragma unroll
+            for (int k = 1; k < T::NBOX; ++k)
+                if (p >= T::start(k)) {
+                    b = k;
+                    q = p - T::start(k);
+                }
+            int x, y, z;
+            // decode q inside box b (boxes are small constant shapes)
+            const int nx = T::box[b][1] - T::box[b][0];
+            const int ny = T::box[b][3] - T::box[b][2];
+            x = T::box[b][0] + q % nx;
+            y = T::box[b][2] + (q / nx) % ny;
+            z = T::box[b][4] + q / (nx * ny);
+            const int gx = ox + x, gy = oy + y, gz = oz + z;
+            const bool ok = p < T::TOTAL && gx < nt0 && gy < nt1 && gz < nt2;
+            loff[i] = ok ? T::lds(x, y, z) : T::NLDS;
+            goff[i] = ok ? gx + nt0 * (gy + nt1 * gz) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) prefetch(k);
+    }
+
+    __device__ __forceinline__ void prefetch(int pos) {
+        const int v = FIELD_ORDER[pos];
+        const int slot = pos % PF;
+        const double* f = in + v * stride;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) buf[slot][i] = f[goff[i]];
+        if (MODE != MODE_RHS) unb[slot] = un[v * stride + cidx];
+    }
+
+    // ---- Stager
+    __device__ double centre(int v) const { return in[v * stride + cidx]; }
+    __device__ FixedStrides<1, T::BX, T::BX * T::BY> strides() const { return {}; }
+    __device__ const double* field(int v) {
+        const int pos = field_pos(v);
+        const int slot = pos % PF;
+        __syncthreads(); // every thread is done with the previous field's tile
+#pragma unroll
+        for (int i = 0; i < NL; ++i) lds[loff[i]] = buf[slot][i];
+        if (MODE != MODE_RHS) unv[v] = unb[slot];
+        __syncthreads();
+        if (pos + PF < NV) prefetch(pos + PF);
+        return lds + cl;
+    }
+
+    // ---- Sink
+    __device__ __forceinline__ void put(int v, double r, double ko, double uin) {
+        const double rr = ko_on ? r + ko : r;
+        if (MODE == MODE_RHS) {
+            if (active) out[v * stride + cidx] = rr;
+            return;
+        }
+        double val = combine_exact(cf.a1, unv[v], cf.a2, uin, cf.a3dt, rr);
+        if (v == CHI) val = floor_chi(val);
+        if (v == ALPHA) val = floor_alpha(val);
+        if (MODE == MODE_FINAL && v >= GXX && v < GXX + 6) {
+            hg[v - GXX] = val;
+            return;
+        }
+        if (MODE == MODE_FINAL && v >= AXX && v < AXX + 6) {
+            hA[v - AXX] = val;
+            return;
+        }
+        if (active) out[v * stride + cidx] = val;
+    }
+
+    __device__ __forceinline__ void finish() {
+        if (MODE != MODE_FINAL) return;
+        algebraic_exact(hg, hA);
+        if (!active) return;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            out[(GXX + a) * stride + cidx] = hg[a];
+            out[(AXX + a) * stride + cidx] = hA[a];
+        }
+    }
+};
+
+struct StageArgs {
+    const double* in;
+    const double* un;
+    double* out;
+    Sources src;
+    int n[3];
+    int nt[3];
+    long stride;
+    int ng;
+    int tiles[3];
+    int ntiles;
+    int xcd_remap;
+    Stencil<double> K;
+    CellParams P;
+    StageCoef cf;
+    int ko_on;
+};
+
+// workgroup id -> tile id: consecutive tiles go to the same XCD (workgroups are dealt
+// round-robin over the 8 XCDs), so that halo rows shared by neighbouring tiles are
+// found in that XCD's L2.
+__device__ __forceinline__ int tile_of_block(int bid, int ntiles, int remap) {
+    if (!remap) return bid;
+    const int per = ntiles >> 3;
+    return (bid & 7) * per + (bid >> 3);
+}
+
+template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
+__global__ __launch_bounds__(TX* TY* TZ) void stage_kernel(StageArgs A) {
+    using T = TileGeom<TX, TY, TZ>;
+    __shared__ double lds[T::NLDS + 1];
+
+    const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
+    const int tix = tile % A.tiles[0];
+    const int tiy = (tile / A.tiles[0]) % A.tiles[1];
+    const int tiz = tile / (A.tiles[0] * A.tiles[1]);
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
+    const int ix = tix * TX + lx, iy = tiy * TY + ly, iz = tiz * TZ + lz; // interior-relative
+    const bool active = ix < A.n[0] && iy < A.n[1] && iz < A.n[2];
+    const int gi = A.ng + (active ? ix : 0), gj = A.ng + (active ? iy : 0), gk = A.ng + (active ? iz : 0);
+    const long cidx = gi + (long)A.nt[0] * (gj + (long)A.nt[1] * gk);
+
+    TileIO<TX, TY, TZ, MODE, 2> io;
+    io.lds = lds;
+    io.in = A.in;
+    io.un = A.un;
+    io.out = A.out;
+    io.stride = A.stride;
+    io.cidx = cidx;
+    io.cl = T::lds(lx, ly, lz);
+    io.active = active;
+    io.ko_on = A.ko_on != 0;
+    io.cf = A.cf;
+    io.init(tid, A.ng + tix * TX, A.ng + tiy * TY, A.ng + tiz * TZ, A.nt[0], A.nt[1], A.nt[2]);
+
+    CellSources src{};
+    if (HAS_SRC) {
+        src.rho = A.src.rho[cidx];
+        src.S = A.src.S[cidx];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) src.Si[a] = A.src.Si[3 * cidx + a];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) src.Sij[a] = A.src.Sij[6 * cidx + a];
+    }
+    ccz4_cell<HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
+}
+
+constexpr int FP64_TX = 8, FP64_TY = 8, FP64_TZ = 4;
+
+template <int MODE, bool HAS_SRC>
+hipError_t launch_stage_t(const StageArgs& A, int nblk, hipStream_t s) {
+    stage_kernel<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, true>
+        <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// ghost fill
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double coord(const BlockDesc& b, int d, int idx) {
+#pragma clang fp contract(off)
+    // GridBlock::x, include/granite/core/grid.hpp:92-94
+    return b.lo[d] + ((double)idx - (double)b.ng + 0.5) * b.dx[d];
+}
+
+__device__ __forceinline__ double radius(double x, double y, double z) {
+#pragma clang fp contract(off)
+    return sqrt(x * x + y * y + z * z) + 1e-30;
+}
+
+__device__ __forceinline__ double f_inf(int v) {
+    return (v == 0 || v == 1 || v == 4 || v == 6 || v == 18) ? 1.0 : 0.0;
+}
+
+// Final value of a ghost cell after the reference's three passes: each pass over axis d
+// rewrites the ghost layers of d over the full transverse extent from the boundary layer,
+// so a cell that is a ghost along a set of axes ends with the chain of 1/r fall-offs from
+// its interior anchor, applied in X, Y, Z order at the intermediate positions
+// (src/main.cpp:395-483).
+__global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode) {
+#pragma clang fp contract(off)
+    const long nA = 2L * b.ng * b.nt[1] * b.nt[0];
+    const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
+    const long nC = (long)b.n[2] * b.n[1] * 2 * b.ng;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nA + nB + nC) return;
+    const int ng = b.ng, g2 = 2 * ng;
+    int i, j, k;
+    if (t < nA) {
+        i = (int)(t % b.nt[0]);
+        j = (int)((t / b.nt[0]) % b.nt[1]);
+        const int kk = (int)(t / ((long)b.nt[0] * b.nt[1]));
+        k = kk < ng ? kk : b.n[2] + kk;
+    } else if (t < nA + nB) {
+        const long q = t - nA;
+        i = (int)(q % b.nt[0]);
+        const int jj = (int)((q / b.nt[0]) % g2);
+        k = ng + (int)(q / ((long)b.nt[0] * g2));
+        j = jj < ng ? jj : b.n[1] + jj;
+    } else {
+        const long q = t - nA - nB;
+        const int ii = (int)(q % g2);
+        j = ng + (int)((q / g2) % b.n[1]);
+        k = ng + (int)(q / ((long)g2 * b.n[1]));
+        i = ii < ng ? ii : b.n[0] + ii;
+    }
+    const int I[3] = {i, j, k};
+    int A[3];
+    bool gh[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        gh[d] = I[d] < ng || I[d] >= ng + b.n[d];
+        if (mode == 1) {
+            int r = (I[d] - ng) % b.n[d];
+            if (r < 0) r += b.n[d];
+            A[d] = ng + r;
+        } else {
+            A[d] = I[d] < ng ? ng : (I[d] >= ng + b.n[d] ? ng + b.n[d] - 1 : I[d]);
+        }
+    }
+    const long dst = I[0] + (long)b.nt[0] * (I[1] + (long)b.nt[1] * I[2]);
+    const long src = A[0] + (long)b.nt[0] * (A[1] + (long)b.nt[1] * A[2]);
+    if (mode != 0) {
+        for (int v = 0; v < NV; ++v) u[v * b.stride + dst] = u[v * b.stride + src];
+        return;
+    }
+    // (r_int, r_gh) of each step of the chain: start at the anchor, move one ghost axis
+    // at a time in X, Y, Z order
+    double rin[3], rgh[3];
+    int nsteps = 0;
+    {
+        double Q[3] = {coord(b, 0, A[0]), coord(b, 1, A[1]), coord(b, 2, A[2])};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (!gh[d]) continue;
+            rin[nsteps] = radius(Q[0], Q[1], Q[2]);
+            Q[d] = coord(b, d, I[d]);
+            rgh[nsteps] = radius(Q[0], Q[1], Q[2]);
+            ++nsteps;
+        }
+    }
+    for (int v = 0; v < NV; ++v) {
+        const double fi = f_inf(v);
+        double val = u[v * b.stride + src];
+        for (int s = 0; s < nsteps; ++s) val = fi + (val - fi) * rin[s] / rgh[s];
+        u[v * b.stride + dst] = val;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Kreiss-Oliger only (applyDissipation), global-memory stencils, reference order
+// ------------------------------------------------------------------------------------
+__global__ void dissipation_kernel(BlockDesc b, const double* __restrict__ u, double* rhs, Stencil<double> K) {
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
+    if (t >= ncell) return;
+    const int i = b.ng + (int)(t % b.n[0]);
+    const int j = b.ng + (int)((t / b.n[0]) % b.n[1]);
+    const int k = b.ng + (int)(t / ((long)b.n[0] * b.n[1]));
+    const long c = i + (long)b.nt[0] * (j + (long)b.nt[1] * k);
+    const long st[3] = {1, b.nt[0], (long)b.nt[0] * b.nt[1]};
+    for (int v = 0; v < NV; ++v) {
+        const double* f = u + v * b.stride + c;
+        double acc = 0.0;
+        for (int d = 0; d < 3; ++d)
+            acc = ko_axis_ordered(acc, K.kw[d], f[-3 * st[d]], f[-2 * st[d]], f[-st[d]], f[0], f[st[d]], f[2 * st[d]],
+                                  f[3 * st[d]]);
+        rhs[v * b.stride + c] += acc;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// constraint monitor (src/spacetime/ccz4.cpp:941-1159), global-memory stencils
+// ------------------------------------------------------------------------------------
+struct GlobalCell {
+    const double* u;
+    long stride, c, st[3];
+    double ih[3], ih2[3], ihh[3];
+    __device__ double f(int v, long off) const { return u[v * stride + c + off]; }
+    __device__ double d1(int v, int d) const {
+        const long q = st[d];
+        return ih[d] * (-f(v, 2 * q) + 8.0 * f(v, q) - 8.0 * f(v, -q) + f(v, -2 * q));
+    }
+    __device__ double d2(int v, int a, int b) const {
+        if (a == b) {
+            const long q = st[a];
+            return ih2[a] * (-f(v, 2 * q) + 16.0 * f(v, q) - 30.0 * f(v, 0) + 16.0 * f(v, -q) - f(v, -2 * q));
+        }
+        const long pa = st[a], pb = st[b];
+        const double h = (a == 0) ? ihh[b == 1 ? 0 : 1] : ihh[2];
+        return h * (f(v, pa + pb) - f(v, pa - pb) - f(v, -pa + pb) + f(v, -pa - pb));
+    }
+};
+
+__global__ void constraints_kernel(BlockDesc b, const double* __restrict__ u, double* ham, double* mom,
+                                   Stencil<double> K) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.stride) return;
+    const int i = (int)(t % b.nt[0]);
+    const int j = (int)((t / b.nt[0]) % b.nt[1]);
+    const int k = (int)(t / ((long)b.nt[0] * b.nt[1]));
+    const bool inner = i >= b.ng && i < b.ng + b.n[0] && j >= b.ng && j < b.ng + b.n[1] && k >= b.ng &&
+        k < b.ng + b.n[2];
+    if (!inner) {
+        ham[t] = 0.0;
+        if (mom) mom[3 * t] = mom[3 * t + 1] = mom[3 * t + 2] = 0.0;
+        return;
+    }
+    GlobalCell C;
+    C.u = u;
+    C.stride = b.stride;
+    C.c = t;
+    C.st[0] = 1;
+    C.st[1] = b.nt[0];
+    C.st[2] = (long)b.nt[0] * b.nt[1];
+    for (int d = 0; d < 3; ++d) {
+        C.ih[d] = K.ih[d];
+        C.ih2[d] = K.ih2[d];
+        C.ihh[d] = K.ihh[d];
+    }
+    const double chi = C.f(CHI, 0);
+    const double Ktr = C.f(KTR, 0);
+    double gt[6], At[6], Gh[3];
+    for (int a = 0; a < 6; ++a) {
+        gt[a] = C.f(GXX + a, 0);
+        At[a] = C.f(AXX + a, 0);
+    }
+    for (int a = 0; a < 3; ++a) Gh[a] = C.f(GHX + a, 0);
+    const double det = gt[0] * (gt[3] * gt[5] - gt[4] * gt[4]) - gt[1] * (gt[1] * gt[5] - gt[4] * gt[2]) +
+        gt[2] * (gt[1] * gt[4] - gt[3] * gt[2]);
+    const double idet = 1.0 / (det + 1.0e-30);
+    const double gu[6] = {(gt[3] * gt[5] - gt[4] * gt[4]) * idet, (gt[2] * gt[4] - gt[1] * gt[5]) * idet,
+                          (gt[1] * gt[4] - gt[2] * gt[3]) * idet, (gt[0] * gt[5] - gt[2] * gt[2]) * idet,
+                          (gt[1] * gt[2] - gt[0] * gt[4]) * idet, (gt[0] * gt[3] - gt[1] * gt[1]) * idet};
+    double Au[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c)
+            Au[a][c] = gu[sym(a, 0)] * At[sym(0, c)] + gu[sym(a, 1)] * At[sym(1, c)] + gu[sym(a, 2)] * At[sym(2, c)];
+    double AA = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) AA += Au[a][c] * Au[c][a];
+    double dgt[6][3], ddgt[6][6], dchi[3], ddchi[6], dGh[3][3];
+    for (int v = 0; v < 6; ++v)
+        for (int d = 0; d < 3; ++d) {
+            dgt[v][d] = C.d1(GXX + v, d);
+            for (int e = d; e < 3; ++e) ddgt[v][sym(d, e)] = C.d2(GXX + v, d, e);
+        }
+    for (int d = 0; d < 3; ++d) {
+        dchi[d] = C.d1(CHI, d);
+        for (int e = d; e < 3; ++e) ddchi[sym(d, e)] = C.d2(CHI, d, e);
+        for (int a = 0; a < 3; ++a) dGh[a][d] = C.d1(GHX + a, d);
+    }
+    double chr[3][6];
+    for (int q = 0; q < 3; ++q)
+        for (int p = 0; p < 6; ++p) {
+            const int a = sym_a(p), c = sym_b(p);
+            double s = 0.0;
+            for (int m = 0; m < 3; ++m) s += gu[sym(q, m)] * (dgt[sym(m, a)][c] + dgt[sym(m, c)][a] - dgt[p][m]);
+            chr[q][p] = 0.5 * s;
+        }
+    double G[3][6];
+    for (int q = 0; q < 3; ++q)
+        for (int p = 0; p < 6; ++p) G[q][p] = gt[sym(q, 0)] * chr[0][p] + gt[sym(q, 1)] * chr[1][p] + gt[sym(q, 2)] * chr[2][p];
+    double Gd[3];
+    for (int q = 0; q < 3; ++q) {
+        double s = 0.0;
+        for (int p = 0; p < 6; ++p) s += ((p == 0 || p == 3 || p == 5) ? 1.0 : 2.0) * gu[p] * chr[q][p];
+        Gd[q] = s;
+    }
+    double lapchi = 0.0, gradchi2 = 0.0;
+    for (int p = 0; p < 6; ++p) {
+        const int l = sym_a(p), m = sym_b(p);
+        const double f = (l == m) ? 1.0 : 2.0;
+        lapchi += f * gu[p] * (ddchi[p] - chr[0][p] * dchi[0] - chr[1][p] * dchi[1] - chr[2][p] * dchi[2]);
+        gradchi2 += f * gu[p] * dchi[l] * dchi[m];
+    }
+    double Rs = 0.0;
+    for (int p = 0; p < 6; ++p) {
+        const int a = sym_a(p), c = sym_b(p);
+        double R = 0.0;
+        for (int q = 0; q < 6; ++q) R += ((q == 0 || q == 3 || q == 5) ? -0.5 : -1.0) * gu[q] * ddgt[p][q];
+        for (int q = 0; q < 3; ++q) {
+            R += 0.5 * (gt[sym(q, a)] * dGh[q][c] + gt[sym(q, c)] * dGh[q][a]);
+            R += (Gh[q] - Gd[q]) * G[q][p];
+            for (int l = 0; l < 3; ++l) {
+                const double Hc = gu[sym(l, 0)] * G[q][sym(c, 0)] + gu[sym(l, 1)] * G[q][sym(c, 1)] + gu[sym(l, 2)] * G[q][sym(c, 2)];
+                const double Ha = gu[sym(l, 0)] * G[q][sym(a, 0)] + gu[sym(l, 1)] * G[q][sym(a, 1)] + gu[sym(l, 2)] * G[q][sym(a, 2)];
+                R += chr[q][sym(l, a)] * Hc + chr[q][sym(l, c)] * Ha;
+            }
+        }
+        const double Dab = ddchi[p] - chr[0][p] * dchi[0] - chr[1][p] * dchi[1] - chr[2][p] * dchi[2];
+        const double Rc = (0.5 / (chi + 1e-30)) * (Dab + gt[p] * lapchi) -
+            (0.25 / ((chi * chi) + 1e-30)) * (dchi[a] * dchi[c] - 3.0 * gt[p] * gradchi2);
+        Rs += ((p == 0 || p == 3 || p == 5) ? 1.0 : 2.0) * gu[p] * (R + Rc);
+    }
+    Rs *= chi;
+    ham[t] = Rs + (2.0 / 3.0) * Ktr * Ktr - AA;
+    if (!mom) return;
+    double dAt[6][3], dK[3];
+    for (int v = 0; v < 6; ++v)
+        for (int d = 0; d < 3; ++d) dAt[v][d] = C.d1(AXX + v, d);
+    for (int d = 0; d < 3; ++d) dK[d] = C.d1(KTR, d);
+    for (int a = 0; a < 3; ++a) {
+        double M = 0.0;
+        for (int c = 0; c < 3; ++c)
+            for (int q = 0; q < 3; ++q) {
+                const double gcq = gu[sym(c, q)];
+                double DA = dAt[sym(q, a)][c];
+                for (int m = 0; m < 3; ++m) DA -= chr[m][sym(c, q)] * At[sym(m, a)] + chr[m][sym(c, a)] * At[sym(q, m)];
+                M += gcq * DA;
+                M -= (3.0 / (2.0 * chi + 1e-30)) * At[sym(a, c)] * gcq * dchi[q];
+            }
+        M -= (2.0 / 3.0) * dK[a];
+        mom[3 * t + a] = M;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// initial data and noise
+// ------------------------------------------------------------------------------------
+__global__ void init_bl_kernel(BlockDesc b, double* u, int nbh, const double* mass, const double* pos) {
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.stride) return;
+    const int i = (int)(t % b.nt[0]);
+    const int j = (int)((t / b.nt[0]) % b.nt[1]);
+    const int k = (int)(t / ((long)b.nt[0] * b.nt[1]));
+    const double x = coord(b, 0, i), y = coord(b, 1, j), z = coord(b, 2, k);
+    // flat background (src/spacetime/ccz4.cpp:1165-1188) then the BL conformal factor
+    // (src/initial_data/initial_data.cpp:79-158)
+    for (int v = 0; v < NV; ++v) u[v * b.stride + t] = 0.0;
+    u[(GXX + 0) * b.stride + t] = 1.0;
+    u[(GXX + 3) * b.stride + t] = 1.0;
+    u[(GXX + 5) * b.stride + t] = 1.0;
+    double psi = 1.0;
+    for (int h = 0; h < nbh; ++h) {
+        const double ax = x - pos[3 * h], ay = y - pos[3 * h + 1], az = z - pos[3 * h + 2];
+        const double r = sqrt(ax * ax + ay * ay + az * az + 1.0e-30);
+        psi += mass[h] / (2.0 * r);
+    }
+    const double p2 = psi * psi;
+    const double chi = 1.0 / (p2 * p2);
+    u[CHI * b.stride + t] = chi;
+    u[ALPHA * b.stride + t] = sqrt(chi);
+    double dp[3] = {0.0, 0.0, 0.0};
+    for (int h = 0; h < nbh; ++h) {
+        const double rx = x - pos[3 * h], ry = y - pos[3 * h + 1], rz = z - pos[3 * h + 2];
+        const double r2 = rx * rx + ry * ry + rz * rz + 1.0e-20;
+        const double r3 = r2 * sqrt(r2);
+        const double c = -mass[h] / (2.0 * r3);
+        dp[0] += c * rx;
+        dp[1] += c * ry;
+        dp[2] += c * rz;
+    }
+    const double gf = 8.0 / (3.0 * psi);
+    for (int a = 0; a < 3; ++a) u[(GHX + a) * b.stride + t] = gf * dp[a];
+}
+
+__device__ __forceinline__ uint64_t sm64(uint64_t s) {
+    s += 0x9E3779B97F4A7C15ull;
+    s = (s ^ (s >> 30)) * 0xBF58476D1CE4E5B9ull;
+    s = (s ^ (s >> 27)) * 0x94D049BB133111EBull;
+    return s ^ (s >> 31);
+}
+
+__global__ void add_noise_kernel(BlockDesc b, double* u, uint64_t seed, double amp, unsigned mask) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.stride) return;
+    for (int v = 0; v < NV; ++v) {
+        if (!((mask >> v) & 1u)) continue;
+        const uint64_t q = (uint64_t)v * (uint64_t)b.stride + (uint64_t)t;
+        const uint64_t z1 = sm64(seed + (2 * q) * 0x9E3779B97F4A7C15ull);
+        const uint64_t z2 = sm64(seed + (2 * q + 1) * 0x9E3779B97F4A7C15ull);
+        const double u1 = ((double)(z1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = (double)(z2 >> 11) * (1.0 / 9007199254740992.0);
+        u[v * b.stride + t] += amp * (sqrt(-2.0 * log(u1)) * cos(2.0 * PI * u2));
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// reductions (fixed order: per-workgroup tree, then one workgroup over the partials)
+// ------------------------------------------------------------------------------------
+constexpr int RED_T = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = RED_T / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__global__ __launch_bounds__(RED_T) void sumsq_kernel(BlockDesc b, const double* ham, double* partial, long per) {
+    __shared__ double sh[RED_T];
+    const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
+    const long beg = (long)blockIdx.x * per;
+    const long end = beg + per < ncell ? beg + per : ncell;
+    double acc = 0.0;
+    for (long t = beg + threadIdx.x; t < end; t += RED_T) {
+        const int i = b.ng + (int)(t % b.n[0]);
+        const int j = b.ng + (int)((t / b.n[0]) % b.n[1]);
+        const int k = b.ng + (int)(t / ((long)b.n[0] * b.n[1]));
+        const double h = ham[i + (long)b.nt[0] * (j + (long)b.nt[1] * k)];
+        acc += h * h;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_T) void nonfinite_kernel(BlockDesc b, const double* u, double* partial, long per) {
+    __shared__ double sh[RED_T];
+    const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
+    const long beg = (long)blockIdx.x * per;
+    const long end = beg + per < ncell ? beg + per : ncell;
+    double acc = 0.0;
+    for (long t = beg + threadIdx.x; t < end; t += RED_T) {
+        const int i = b.ng + (int)(t % b.n[0]);
+        const int j = b.ng + (int)((t / b.n[0]) % b.n[1]);
+        const int k = b.ng + (int)(t / ((long)b.n[0] * b.n[1]));
+        const long c = i + (long)b.nt[0] * (j + (long)b.nt[1] * k);
+        for (int v = 0; v < NV; ++v) acc += __builtin_isfinite(u[v * b.stride + c]) ? 0.0 : 1.0;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_T) void sum_partials_kernel(const double* partial, int n, double* out) {
+    __shared__ double sh[RED_T];
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < n; t += RED_T) acc += partial[t];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+inline unsigned grid_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+} // namespace
+
+Stencil<double> make_stencil(const BlockDesc& b, double ko_sigma) {
+    Stencil<double> K;
+    static const double wts[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+    for (int d = 0; d < 3; ++d) {
+        K.ih[d] = 1.0 / (12.0 * b.dx[d]);
+        K.ih2[d] = 1.0 / (12.0 * b.dx[d] * b.dx[d]);
+        K.ko[d] = ko_sigma / (64.0 * b.dx[d]);
+        for (int s = 0; s < 7; ++s) K.kw[d][s] = K.ko[d] * wts[s];
+    }
+    K.ihh[0] = 1.0 / (4.0 * b.dx[0] * b.dx[1]);
+    K.ihh[1] = 1.0 / (4.0 * b.dx[0] * b.dx[2]);
+    K.ihh[2] = 1.0 / (4.0 * b.dx[1] * b.dx[2]);
+    return K;
+}
+
+void stage_tile_fp64(int* t3) {
+    t3[0] = FP64_TX;
+    t3[1] = FP64_TY;
+    t3[2] = FP64_TZ;
+}
+
+hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
+                             const CellParams& P, double ko_sigma, int mode, const StageCoef& cf, hipStream_t s) {
+    StageArgs A;
+    A.in = in;
+    A.un = un;
+    A.out = out;
+    A.src = src;
+    for (int d = 0; d < 3; ++d) {
+        A.n[d] = b.n[d];
+        A.nt[d] = b.nt[d];
+    }
+    A.stride = b.stride;
+    A.ng = b.ng;
+    A.tiles[0] = (b.n[0] + FP64_TX - 1) / FP64_TX;
+    A.tiles[1] = (b.n[1] + FP64_TY - 1) / FP64_TY;
+    A.tiles[2] = (b.n[2] + FP64_TZ - 1) / FP64_TZ;
+    A.ntiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    A.xcd_remap = (A.ntiles % 8 == 0) ? 1 : 0;
+    A.K = make_stencil(b, ko_sigma);
+    A.P = P;
+    A.cf = cf;
+    A.ko_on = ko_sigma > 0.0 ? 1 : 0;
+    const bool has_src = src.rho != nullptr;
+    if (has_src) {
+        if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, true>(A, A.ntiles, s);
+        if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, true>(A, A.ntiles, s);
+        return launch_stage_t<MODE_FINAL, true>(A, A.ntiles, s);
+    }
+    if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, false>(A, A.ntiles, s);
+    if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, false>(A, A.ntiles, s);
+    return launch_stage_t<MODE_FINAL, false>(A, A.ntiles, s);
+}
+
+hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s) {
+    const long nA = 2L * b.ng * b.nt[1] * b.nt[0];
+    const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
+    const long nC = (long)b.n[2] * b.n[1] * 2 * b.ng;
+    const long n = nA + nB + nC;
+    fill_ghosts_kernel<<<grid_for(n, 256), 256, 0, s>>>(b, u, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_dissipation(const BlockDesc& b, const double* u, double* rhs, double ko_sigma, hipStream_t s) {
+    if (!(ko_sigma > 0.0)) return hipSuccess;
+    const long n = (long)b.n[0] * b.n[1] * b.n[2];
+    dissipation_kernel<<<grid_for(n, 256), 256, 0, s>>>(b, u, rhs, make_stencil(b, ko_sigma));
+    return hipGetLastError();
+}
+
+hipError_t launch_constraints(const BlockDesc& b, const double* u, double* ham, double* mom, hipStream_t s) {
+    constraints_kernel<<<grid_for(b.stride, 128), 128, 0, s>>>(b, u, ham, mom,
+                                                              make_stencil(b, 0.0));
+    return hipGetLastError();
+}
+
+hipError_t launch_init_bl(const BlockDesc& b, double* u, int nbh, const double* mass, const double* pos,
+                          hipStream_t s) {
+    init_bl_kernel<<<grid_for(b.stride, 256), 256, 0, s>>>(b, u, nbh, mass, pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_noise(const BlockDesc& b, double* u, unsigned long long seed, double amp, unsigned mask,
+                            hipStream_t s) {
+    add_noise_kernel<<<grid_for(b.stride, 256), 256, 0, s>>>(b, u, seed, amp, mask);
+    return hipGetLastError();
+}
+
+static long reduce_split(const BlockDesc& b, int cap, int* nparts) {
+    const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
+    long per = (ncell + cap - 1) / cap;
+    if (per < RED_T) per = RED_T;
+    *nparts = (int)((ncell + per - 1) / per);
+    return per;
+}
+
+hipError_t launch_sumsq_interior(const BlockDesc& b, const double* ham, double* partial, int cap, int* nparts,
+                                 hipStream_t s) {
+    const long per = reduce_split(b, cap, nparts);
+    sumsq_kernel<<<*nparts, RED_T, 0, s>>>(b, ham, partial, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_nonfinite(const BlockDesc& b, const double* u, double* partial, int cap, int* nparts,
+                                  hipStream_t s) {
+    const long per = reduce_split(b, cap, nparts);
+    nonfinite_kernel<<<*nparts, RED_T, 0, s>>>(b, u, partial, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t s) {
+    sum_partials_kernel<<<1, RED_T, 0, s>>>(partial, n, out);
+    return hipGetLastError();
+}
+
+} // namespace gr

@@ -1,0 +1,73 @@
+// Internal launch interface between the C ABI (capi.hip) and the gfx950 kernels
+// (kernels.hip). Not part of the public boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ccz4_cell.hpp"
+
+namespace gr {
+
+// one block as the kernels see it
+struct BlockDesc {
+    int n[3];
+    int nt[3];
+    int ng;
+    long stride;
+    double lo[3];
+    double dx[3];
+};
+
+enum StageMode : int { MODE_RHS = 0, MODE_STAGE = 1, MODE_FINAL = 2 };
+
+struct StageCoef {
+    double a1;   // weight of u^n (read pointwise from `un`)
+    double a2;   // weight of the stage input (centre of `in`)
+    double a3dt; // a3 * dt, weight of the RHS
+};
+
+struct Sources {
+    const double* rho;
+    const double* Si;
+    const double* Sij;
+    const double* S;
+};
+
+// stencil constants for a block, with the reference's operation order
+Stencil<double> make_stencil(const BlockDesc& b, double ko_sigma);
+
+// RHS / RK3 stage kernel on one block (FP64). in: stencil input (ghosts filled);
+// un: u^n for the combine (ignored for MODE_RHS); out: RHS (MODE_RHS) or the new state.
+hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
+                             const CellParams& P, double ko_sigma, int mode, const StageCoef& cf, hipStream_t s);
+
+// Ghost fill of one block: mode 0 Sommerfeld (chained X,Y,Z), 1 periodic, 2 outflow copy.
+hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s);
+
+// rhs += Kreiss-Oliger(u) on the interior (reference accumulation order)
+hipError_t launch_dissipation(const BlockDesc& b, const double* u, double* rhs, double ko_sigma, hipStream_t s);
+
+// H and M_i on interior cells; ghosts of ham/mom are zeroed. mom may be null.
+hipError_t launch_constraints(const BlockDesc& b, const double* u, double* ham, double* mom, hipStream_t s);
+
+// Brill-Lindquist initial data on all cells. mass/pos: device arrays (nbh, 3*nbh).
+hipError_t launch_init_bl(const BlockDesc& b, double* u, int nbh, const double* mass, const double* pos,
+                          hipStream_t s);
+
+hipError_t launch_add_noise(const BlockDesc& b, double* u, unsigned long long seed, double amp, unsigned mask,
+                            hipStream_t s);
+
+// partial[blockIdx] = sum over interior of ham^2 ; count is analytic. Returns the number of
+// partials written in *nparts (<= cap).
+hipError_t launch_sumsq_interior(const BlockDesc& b, const double* ham, double* partial, int cap, int* nparts,
+                                 hipStream_t s);
+// sums `n` doubles into out[0] with one workgroup (fixed order)
+hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
+// partial[blockIdx] = number of non-finite interior values over all 22 fields
+hipError_t launch_count_nonfinite(const BlockDesc& b, const double* u, double* partial, int cap, int* nparts,
+                                  hipStream_t s);
+
+// tile shape of the FP64 stage kernel
+void stage_tile_fp64(int* t3);
+
+} // namespace gr

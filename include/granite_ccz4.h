@@ -1,0 +1,179 @@
+/*
+ * granite_ccz4.h - C ABI of the MI355X CCZ4 right-hand-side and SSP-RK3 stage engine.
+ *
+ * This is the drop-in boundary for GRANITE's CPU path. Every entry point names the
+ * reference interface it replaces. Conventions:
+ *   - Storage of one block follows GridBlock (include/granite/core/grid.hpp:28-42,157-161):
+ *     u[v*stride + i + nx*(j + ny*k)], nx = n[0]+2*ng, ny = n[1]+2*ng, nz = n[2]+2*ng,
+ *     stride = nx*ny*nz, v in 0..21 in SpacetimeVar order (include/granite/core/types.hpp:104-127).
+ *   - Host pointers passed in are borrowed for the duration of the call only.
+ *   - Every call returns int: 0 = GR_OK, > 0 = a hipError_t, < 0 = an engine error (GR_ERR_*).
+ *     gr_last_error() gives the text of the last failure on that engine.
+ *   - One host thread per engine; all work of an engine runs in order on one HIP stream.
+ *   - No torch, HIP or C++ types cross this boundary.
+ */
+#ifndef GRANITE_CCZ4_H
+#define GRANITE_CCZ4_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GR_NUM_VARS 22
+
+enum gr_status {
+    GR_OK = 0,
+    GR_ERR_ARG = -1,      /* null pointer or out-of-range argument */
+    GR_ERR_GEOM = -2,     /* bad block geometry (n < 1, ng < 4, non-positive dx) */
+    GR_ERR_NVARS = -3,    /* a variable count other than 22 */
+    GR_ERR_NODEV = -4,    /* no usable gfx950 device / extension not initialised */
+    GR_ERR_STATE = -5,    /* call not valid in the current state */
+    GR_ERR_NONFINITE = -6 /* (reported by gr_level_nonfinite only) */
+};
+
+/* Ghost-fill modes. SOMMERFELD is the reference's outer BC after every RK stage
+ * (src/main.cpp:351-521); OUTFLOW is the level-0 zero-gradient fill (src/amr/amr.cpp:436-480);
+ * PERIODIC has no reference counterpart (used for the periodic gauge-wave box). */
+enum gr_bc_mode { GR_BC_NONE = -1, GR_BC_SOMMERFELD = 0, GR_BC_PERIODIC = 1, GR_BC_OUTFLOW = 2 };
+
+/* Arithmetic of the RK3 stage kernels. FP64 is the parity form. MIXED evaluates the
+ * stencils and RHS algebra in FP32 and keeps FP64 in the RK3 state and combine, in
+ * chi and the lapse, in the conformal-metric determinant/inverse and in the constraint
+ * monitor. */
+enum gr_precision { GR_PREC_FP64 = 0, GR_PREC_MIXED = 1 };
+
+/* Block geometry as GridBlock holds it: n interior cells per axis, ng ghosts,
+ * lo = lower corner of the interior, dx = (hi - lo) / n per axis (src/core/grid.cpp:22-24). */
+typedef struct gr_geom {
+    int n[3];
+    int ng;
+    double lo[3];
+    double dx[3];
+} gr_geom;
+
+/* CCZ4Params (include/granite/spacetime/ccz4.hpp:25-34); fd_order and
+ * use_moving_puncture are not read by the reference RHS and are not carried. */
+typedef struct gr_ccz4_params {
+    double kappa1;
+    double kappa2;
+    double eta;
+    double ko_sigma;
+    double chi_blend_center;
+    double chi_blend_width;
+} gr_ccz4_params;
+
+typedef struct gr_engine gr_engine;
+typedef struct gr_level gr_level;
+
+/* ---- engine ------------------------------------------------------------------- */
+
+/* Create an engine on HIP device `device` (its own stream, its kernels loaded).
+ * Replaces: construction of spacetime::CCZ4Evolution (include/granite/spacetime/ccz4.hpp:51)
+ * as far as device state is concerned; parameters travel with each call. */
+int gr_engine_create(int device, gr_engine** out);
+int gr_engine_destroy(gr_engine* eng);
+/* Text of the last error on this engine (static storage owned by the engine). */
+const char* gr_last_error(const gr_engine* eng);
+/* Wait for all work queued on the engine's stream. */
+int gr_engine_sync(gr_engine* eng);
+
+/* ---- per-block calls on host memory (compatibility mode) ---------------------------- */
+
+/* CCZ4 RHS plus Kreiss-Oliger dissipation on the interior of one block.
+ * Replaces: CCZ4Evolution::computeRHS (src/spacetime/ccz4.cpp:251-865, declared
+ * include/granite/spacetime/ccz4.hpp:69-74) and, with all four source pointers null,
+ * CCZ4Evolution::computeRHSVacuum (src/spacetime/ccz4.cpp:229-245).
+ * u: 22*stride doubles (ghosts filled). rhs: 22*stride doubles; only interior cells are
+ * written, ghost cells keep their values. Sources (each totalSize = stride cells, all
+ * null or all non-null): rho[stride], Si[3*stride] (AoS), Sij[6*stride] (AoS, xx,xy,xz,yy,yz,zz),
+ * S[stride]. nvars must be 22. */
+int gr_ccz4_rhs(gr_engine* eng, const gr_geom* geom, const gr_ccz4_params* prm, int nvars, const double* u,
+                double* rhs, const double* rho, const double* Si, const double* Sij, const double* S);
+
+/* Adds the Kreiss-Oliger term to rhs (interior); no-op when ko_sigma <= 0.
+ * Replaces: CCZ4Evolution::applyDissipation (src/spacetime/ccz4.cpp:871-935). */
+int gr_ccz4_dissipation(gr_engine* eng, const gr_geom* geom, const gr_ccz4_params* prm, int nvars,
+                        const double* u, double* rhs);
+
+/* Hamiltonian and momentum constraints on interior cells; ghost cells of ham/mom are 0.
+ * ham: stride doubles; mom: 3*stride doubles (AoS).
+ * Replaces: CCZ4Evolution::computeConstraints (src/spacetime/ccz4.cpp:941-1159). */
+int gr_ccz4_constraints(gr_engine* eng, const gr_geom* geom, const double* u, double* ham, double* mom);
+
+/* ---- per-block calls on device memory ------------------------------------------------ */
+
+/* As gr_ccz4_rhs, but every pointer is device memory of this engine's device and the
+ * call is asynchronous on the engine's stream. */
+int gr_ccz4_rhs_device(gr_engine* eng, const gr_geom* geom, const gr_ccz4_params* prm, const double* d_u,
+                       double* d_rhs, const double* d_rho, const double* d_Si, const double* d_Sij,
+                       const double* d_S);
+
+/* ---- a refinement level held on the device -------------------------------------------- */
+
+/* A set of blocks evolved together (the blocks the reference passes to its per-level
+ * callback EvolutionStepFunc, include/granite/amr/amr.hpp:43; body src/main.cpp:1142-1177).
+ * Device memory per block: state u plus two stage buffers, 3 * 22 * stride doubles. */
+int gr_level_create(gr_engine* eng, int nblocks, const gr_geom* geoms, gr_level** out);
+int gr_level_destroy(gr_level* lvl);
+int gr_level_upload(gr_level* lvl, int block, const double* host_u);
+int gr_level_download(gr_level* lvl, int block, double* host_u);
+/* Device address of block `block`'s state (22*stride doubles) for device-resident callers. */
+int gr_level_state(gr_level* lvl, int block, double** d_u);
+
+/* Brill-Lindquist data on every cell of every block, ghosts included
+ * (src/initial_data/initial_data.cpp:97-158). pos: 3*nbh doubles. */
+int gr_level_init_brill_lindquist(gr_level* lvl, int nbh, const double* mass, const double* pos);
+/* Adds amp * N(0,1) to each field whose bit is set in field_mask, on every cell. The
+ * normal deviate for field v at flat cell p is a function of (seed, v*stride + p) only:
+ * splitmix64 at counters 2q and 2q+1, then Box-Muller (cosine branch). */
+int gr_level_add_noise(gr_level* lvl, uint64_t seed, double amp, uint32_t field_mask);
+
+/* Fill the ghost cells of every block's state with `mode`. */
+int gr_level_fill_ghosts(gr_level* lvl, int mode);
+
+/* One SSP-RK3 step of every block of the level, in the reference order of
+ * TimeIntegrator::sspRK3Step (src/main.cpp:523-689): three (RHS + dissipation + stage
+ * combine + floors) stages, the algebraic constraints after the last, and the ghost
+ * fill `bc_mode` after each stage (GR_BC_SOMMERFELD reproduces the reference).
+ * pre_fill (a gr_bc_mode or GR_BC_NONE) is applied to the state before the step, as the
+ * reference's AMRHierarchy::subcycle does on level 0 (src/amr/amr.cpp:319).
+ * use_graph != 0 replays the step from a captured HIP graph (captured on first use for a
+ * given (dt, params, modes)). Asynchronous: ordered on the engine stream. */
+int gr_level_rk3_step(gr_level* lvl, const gr_ccz4_params* prm, double dt, int precision, int bc_mode,
+                      int pre_fill, int use_graph);
+
+/* Wait for the level's queued work. */
+int gr_level_sync(gr_level* lvl);
+
+/* Sum of H^2 over the interiors of all blocks and the number of cells summed, with the
+ * FP64 monitor (the reference prints sqrt(sumsq/count), src/main.cpp:1216-1250). */
+int gr_level_constraints_l2(gr_level* lvl, double* sumsq, long long* count);
+
+/* Number of non-finite values in the interiors of all blocks' state (NaN scan of
+ * src/main.cpp:1192-1214). */
+int gr_level_nonfinite(gr_level* lvl, long long* count);
+
+/* ---- measurement (no reference counterpart) ----------------------------------------------- */
+
+/* enable != 0: every RK3 step run without a graph records HIP events around each of its
+ * kernels on the engine stream (graph replay is not instrumented). */
+int gr_level_profile(gr_level* lvl, int enable);
+/* Waits for the stream, then returns and clears the accumulated device time of one kernel
+ * kind since the last read: kind 0 = stage kernels of RK3 stages 1-2, 1 = the last-stage
+ * kernel (with the algebraic constraints), 2 = ghost fills. */
+int gr_level_profile_read(gr_level* lvl, int kind, double* total_ms, int* launches);
+
+/* ---- introspection ------------------------------------------------------------------- */
+
+/* Stage-kernel tile shape in cells (x, y, z) used for the given precision. */
+int gr_stage_tile(int precision, int* tile3);
+/* Library version string. */
+const char* gr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRANITE_CCZ4_H */

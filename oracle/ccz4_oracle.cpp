@@ -1,0 +1,935 @@
+// CPU oracle for the CCZ4 right-hand side and SSP-RK3 stage path.
+//
+// TEST INFRASTRUCTURE ONLY. Nothing in the product (granite_amd/) links, loads or
+// calls this file; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg use it, and only as the checker or the timed CPU baseline.
+//
+// It restates, in plain double precision with IEEE semantics (built with
+// -ffp-contract=off -fno-fast-math), the algorithm of GRANITE's CPU path:
+//   rhs            src/spacetime/ccz4.cpp:251-865   (CCZ4Evolution::computeRHS)
+//   d1/d1up/d2     src/spacetime/ccz4.cpp:71-223
+//   ko             src/spacetime/ccz4.cpp:871-935   (applyDissipation)
+//   constraints    src/spacetime/ccz4.cpp:941-1159  (computeConstraints)
+//   flat / gauge   src/spacetime/ccz4.cpp:1165-1221 (setFlatSpacetime, setGaugeWaveData)
+//   combine        src/main.cpp:306-335
+//   sommerfeld     src/main.cpp:351-521
+//   floors         src/main.cpp:588-608
+//   algebraic      src/main.cpp:612-671
+//   rk3 sequence   src/main.cpp:673-689
+//   outflow fill   src/amr/amr.cpp:436-480
+//   prolongate     src/amr/amr.cpp:484-564
+//   restrict       src/amr/amr.cpp:566-626
+//   brill-lindq.   src/initial_data/initial_data.cpp:79-158
+// Storage follows GridBlock (include/granite/core/grid.hpp:28-42,157-161):
+//   u[v*stride + i + nx*(j + ny*k)], nx = n0+2ng, stride = nx*ny*nz.
+// Where the summation order of a term decides the last bits, this file keeps the
+// order the reference uses, so that it agrees with the compiled reference to
+// round-off; every other choice (data structures, loop shapes) is its own.
+//
+// Parity status of each entry point (what pins it) is listed in its comment.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int NV = 22;
+enum : int {
+    F_CHI = 0, F_GXX = 1, F_AXX = 7, F_K = 13, F_GH = 14, F_THETA = 17, F_ALPHA = 18, F_BETA = 19
+};
+constexpr double PI = 3.14159265358979323846;
+
+// symmetric-pair table: (a,b) -> 0..5 in the order xx,xy,xz,yy,yz,zz
+constexpr int SYM[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+
+struct Geom {
+    int n[3];
+    int ng;
+    double lo[3];
+    double dx[3];
+    int nt[3];
+    long stride;
+    long at(int i, int j, int k) const { return i + (long)nt[0] * (j + (long)nt[1] * k); }
+    double x(int d, int idx) const { return lo[d] + ((double)idx - ng + 0.5) * dx[d]; }
+};
+
+Geom make_geom(const int* n, int ng, const double* lo, const double* dx) {
+    Geom g;
+    for (int d = 0; d < 3; ++d) {
+        g.n[d] = n[d];
+        g.lo[d] = lo[d];
+        g.dx[d] = dx[d];
+        g.nt[d] = n[d] + 2 * ng;
+    }
+    g.ng = ng;
+    g.stride = (long)g.nt[0] * g.nt[1] * g.nt[2];
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Stencil evaluator bound to one cell. Offsets are taken along axis `d`.
+// ---------------------------------------------------------------------------
+struct Cell {
+    const Geom& g;
+    const double* u;
+    int i, j, k;
+    long step[3];
+    long c;
+
+    Cell(const Geom& g_, const double* u_, int i_, int j_, int k_) : g(g_), u(u_), i(i_), j(j_), k(k_) {
+        step[0] = 1;
+        step[1] = g.nt[0];
+        step[2] = (long)g.nt[0] * g.nt[1];
+        c = g.at(i, j, k);
+    }
+    double f(int v, long off) const { return u[v * g.stride + c + off]; }
+    double at(int v, int d, int s) const { return f(v, s * step[d]); }
+
+    // 4th-order centred first derivative (ccz4.cpp:71-99)
+    double d1(int v, int d) const {
+        const double s = 1.0 / (12.0 * g.dx[d]);
+        return s * (-at(v, d, 2) + 8.0 * at(v, d, 1) - 8.0 * at(v, d, -1) + at(v, d, -2));
+    }
+    // one-sided 5-point derivative, side chosen by the sign of b (ccz4.cpp:109-152)
+    double d1up(int v, int d, double b) const {
+        const double s = 1.0 / (12.0 * g.dx[d]);
+        if (b >= 0.0)
+            return s * (25.0 * at(v, d, 0) - 48.0 * at(v, d, -1) + 36.0 * at(v, d, -2) -
+                         16.0 * at(v, d, -3) + 3.0 * at(v, d, -4));
+        return s * (-25.0 * at(v, d, 0) + 48.0 * at(v, d, 1) - 36.0 * at(v, d, 2) +
+                    16.0 * at(v, d, 3) - 3.0 * at(v, d, 4));
+    }
+    // second derivative: 4th-order pure, 2nd-order 4-point mixed (ccz4.cpp:154-223)
+    double d2(int v, int a, int b) const {
+        if (a == b) {
+            const double s = 1.0 / (12.0 * g.dx[a] * g.dx[a]);
+            return s * (-at(v, a, 2) + 16.0 * at(v, a, 1) - 30.0 * at(v, a, 0) + 16.0 * at(v, a, -1) -
+                        at(v, a, -2));
+        }
+        const double s = 1.0 / (4.0 * g.dx[a] * g.dx[b]);
+        const long pa = step[a], pb = step[b];
+        return s * (f(v, pa + pb) - f(v, pa - pb) - f(v, -pa + pb) + f(v, -pa - pb));
+    }
+};
+
+struct Params {
+    double kappa1, kappa2, eta, ko_sigma, chi_c, chi_w;
+};
+
+// ---------------------------------------------------------------------------
+// Pointwise RHS at one interior cell (ccz4.cpp:270-857). `out` receives the 22
+// values in field order. Sources may be null (vacuum).
+// ---------------------------------------------------------------------------
+void cell_rhs(const Geom& g, const Params& P, const double* u, int i, int j, int k, const double* rho_a,
+              const double* Si_a, const double* Sij_a, const double* S_a, double* out) {
+    Cell C(g, u, i, j, k);
+    const long flat = C.c;
+
+    const double chi_raw = C.f(F_CHI, 0);
+    const double chi = (std::isfinite(chi_raw) && chi_raw >= 1.0e-4) ? chi_raw : 1.0e-4;
+    const double alpha = C.f(F_ALPHA, 0);
+    const double K = C.f(F_K, 0);
+    const double Theta = C.f(F_THETA, 0);
+    double gt[6], At[6], Gh[3], be[3];
+    for (int a = 0; a < 6; ++a) {
+        gt[a] = C.f(F_GXX + a, 0);
+        At[a] = C.f(F_AXX + a, 0);
+    }
+    for (int a = 0; a < 3; ++a) {
+        Gh[a] = C.f(F_GH + a, 0);
+        be[a] = C.f(F_BETA + a, 0);
+    }
+    const double rho = rho_a ? rho_a[flat] : 0.0;
+    const double Strace = S_a ? S_a[flat] : 0.0;
+    double Si[3] = {0.0, 0.0, 0.0}, Sij[6] = {0, 0, 0, 0, 0, 0};
+    if (Si_a)
+        for (int a = 0; a < 3; ++a) Si[a] = Si_a[3 * flat + a];
+    if (Sij_a)
+        for (int a = 0; a < 6; ++a) Sij[a] = Sij_a[6 * flat + a];
+
+    // inverse conformal metric from cofactors (ccz4.cpp:318-330)
+    const double det = gt[0] * (gt[3] * gt[5] - gt[4] * gt[4]) - gt[1] * (gt[1] * gt[5] - gt[4] * gt[2]) +
+        gt[2] * (gt[1] * gt[4] - gt[3] * gt[2]);
+    const double idet = 1.0 / (det + 1.0e-30);
+    double gu[6];
+    gu[0] = (gt[3] * gt[5] - gt[4] * gt[4]) * idet;
+    gu[1] = (gt[2] * gt[4] - gt[1] * gt[5]) * idet;
+    gu[2] = (gt[1] * gt[4] - gt[2] * gt[3]) * idet;
+    gu[3] = (gt[0] * gt[5] - gt[2] * gt[2]) * idet;
+    gu[4] = (gt[1] * gt[2] - gt[0] * gt[4]) * idet;
+    gu[5] = (gt[0] * gt[3] - gt[1] * gt[1]) * idet;
+
+    // first derivatives [field-component][axis]
+    double dchi[3], dal[3], dK[3], dgt[6][3], dAt[6][3], dGh[3][3], dbe[3][3];
+    for (int d = 0; d < 3; ++d) {
+        dchi[d] = C.d1(F_CHI, d);
+        dal[d] = C.d1(F_ALPHA, d);
+        dK[d] = C.d1(F_K, d);
+        for (int a = 0; a < 6; ++a) {
+            dgt[a][d] = C.d1(F_GXX + a, d);
+            dAt[a][d] = C.d1(F_AXX + a, d);
+        }
+        for (int a = 0; a < 3; ++a) {
+            dGh[a][d] = C.d1(F_GH + a, d);
+            dbe[a][d] = C.d1(F_BETA + a, d);
+        }
+    }
+    // second derivatives, symmetric in the two axes
+    double ddchi[3][3], ddal[3][3], ddgt[6][3][3], ddbe[3][3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) {
+            ddchi[a][b] = ddchi[b][a] = C.d2(F_CHI, a, b);
+            ddal[a][b] = ddal[b][a] = C.d2(F_ALPHA, a, b);
+            for (int c = 0; c < 6; ++c) ddgt[c][a][b] = ddgt[c][b][a] = C.d2(F_GXX + c, a, b);
+            for (int c = 0; c < 3; ++c) ddbe[c][a][b] = ddbe[c][b][a] = C.d2(F_BETA + c, a, b);
+        }
+
+    // Christoffel symbols of the conformal metric, chr[i][sym(j,k)] (ccz4.cpp:413-444)
+    double chr[3][6];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = b; c < 3; ++c) {
+                double s = 0.0;
+                for (int l = 0; l < 3; ++l)
+                    s += 0.5 * gu[SYM[a][l]] * (dgt[SYM[l][c]][b] + dgt[SYM[l][b]][c] - dgt[SYM[b][c]][l]);
+                chr[a][SYM[b][c]] = s;
+            }
+
+    // lowered Christoffel contraction helper: sum_n gt[k n] chr[n][p]
+    auto lowc = [&](int kk, int p) {
+        double s = 0.0;
+        for (int n = 0; n < 3; ++n) s += gt[SYM[kk][n]] * chr[n][p];
+        return s;
+    };
+
+    // grad-chi contractions shared by all six R^chi components (ccz4.cpp:524-535)
+    double lapchi = 0.0, gradchi2 = 0.0;
+    for (int l = 0; l < 3; ++l)
+        for (int m = 0; m < 3; ++m) {
+            const double glm = gu[SYM[l][m]];
+            double Dlm = ddchi[l][m];
+            for (int q = 0; q < 3; ++q) Dlm -= chr[q][SYM[l][m]] * dchi[q];
+            lapchi += glm * Dlm;
+            gradchi2 += glm * dchi[l] * dchi[m];
+        }
+
+    // conformal Ricci R~_ij and conformal-factor part R^chi_ij (ccz4.cpp:465-543)
+    double Rt[6], Rc[6];
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) {
+            const int ab = SYM[a][b];
+            double R = 0.0;
+            for (int l = 0; l < 3; ++l)
+                for (int m = 0; m < 3; ++m) R += -0.5 * gu[SYM[l][m]] * ddgt[ab][l][m];
+            for (int q = 0; q < 3; ++q) R += 0.5 * (gt[SYM[q][a]] * dGh[q][b] + gt[SYM[q][b]] * dGh[q][a]);
+            for (int q = 0; q < 3; ++q) R += Gh[q] * lowc(q, ab);
+            for (int l = 0; l < 3; ++l)
+                for (int m = 0; m < 3; ++m) {
+                    const double glm = gu[SYM[l][m]];
+                    for (int q = 0; q < 3; ++q) {
+                        const double Gb = lowc(q, SYM[b][m]);
+                        const double Ga = lowc(q, SYM[a][m]);
+                        R += glm * (chr[q][SYM[l][a]] * Gb + chr[q][SYM[l][b]] * Ga);
+                        const double Gab = lowc(q, ab);
+                        R -= glm * chr[q][SYM[l][m]] * Gab;
+                    }
+                }
+            Rt[ab] = R;
+
+            double Dab = ddchi[a][b];
+            for (int q = 0; q < 3; ++q) Dab -= chr[q][ab] * dchi[q];
+            const double t1 = (0.5 / (chi + 1e-30)) * (Dab + gt[ab] * lapchi);
+            const double t2 = (0.25 / ((chi * chi) + 1e-30)) * (dchi[a] * dchi[b] - 3.0 * gt[ab] * gradchi2);
+            Rc[ab] = t1 - t2;
+        }
+
+    const double divb = dbe[0][0] + dbe[1][1] + dbe[2][2];
+    const double w = 0.5 * (1.0 + std::tanh((chi - P.chi_c) / P.chi_w));
+
+    // blended advection: w * upwind + (1-w) * centred (ccz4.cpp:563-574)
+    auto advect = [&](int v, const double* dcen) {
+        double up = 0.0, cen = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            up += be[d] * C.d1up(v, d, be[d]);
+            cen += be[d] * dcen[d];
+        }
+        return w * up + (1.0 - w) * cen;
+    };
+    double tmp3[3];
+    auto cen_of = [&](int v) {
+        for (int d = 0; d < 3; ++d) tmp3[d] = C.d1(v, d);
+        return tmp3;
+    };
+
+    // chi (ccz4.cpp:580)
+    out[F_CHI] = (2.0 / 3.0) * chi * (alpha * K - divb) + advect(F_CHI, cen_of(F_CHI));
+
+    // conformal metric (ccz4.cpp:586-607)
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) {
+            const int ab = SYM[a][b];
+            double L = advect(F_GXX + ab, dgt[ab]);
+            for (int q = 0; q < 3; ++q) L += gt[SYM[a][q]] * dbe[q][b] + gt[SYM[b][q]] * dbe[q][a];
+            out[F_GXX + ab] = -2.0 * alpha * At[ab] + L - (2.0 / 3.0) * gt[ab] * divb;
+        }
+
+    // A~_ij A~^ij (ccz4.cpp:609-619)
+    double AA = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < 3; ++d) AA += gu[SYM[a][c]] * gu[SYM[b][d]] * At[SYM[a][b]] * At[SYM[c][d]];
+
+    // lapse Laplacian (ccz4.cpp:621-648)
+    double lap = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) {
+            const double gab = gu[SYM[a][b]];
+            const double fac = (a == b) ? 1.0 : 2.0;
+            lap += fac * gab * ddal[a][b];
+            for (int q = 0; q < 3; ++q) lap -= fac * gab * chr[q][SYM[a][b]] * dal[q];
+        }
+    double cga = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) cga += gu[SYM[a][b]] * dchi[a] * dal[b];
+    const double plap = chi * lap - 0.5 * cga;
+
+    // K (ccz4.cpp:650-652)
+    out[F_K] = -plap + alpha * (AA + K * K / 3.0) + 4.0 * PI * alpha * (rho + Strace) +
+        P.kappa1 * (1.0 - P.kappa2) * alpha * Theta + advect(F_K, dK);
+
+    // Theta (ccz4.cpp:654-665)
+    double Rs = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Rs += chi * gu[SYM[a][b]] * (Rt[SYM[a][b]] + Rc[SYM[a][b]]);
+    out[F_THETA] = 0.5 * alpha * (Rs - AA + 2.0 * K * K / 3.0 - 16.0 * PI * rho) -
+        alpha * P.kappa1 * (2.0 + P.kappa2) * Theta + advect(F_THETA, cen_of(F_THETA));
+
+    // A~_ij: full value, then its trace removed (ccz4.cpp:671-726)
+    double raw[6];
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) {
+            const int ab = SYM[a][b];
+            double DDa = ddal[a][b];
+            for (int q = 0; q < 3; ++q) DDa -= chr[q][ab] * dal[q];
+            const double t1 = chi * (-DDa + alpha * ((Rt[ab] + Rc[ab]) - 8.0 * PI * Sij[ab]));
+            double Asq = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                double Aub = 0.0;
+                for (int l = 0; l < 3; ++l) Aub += gu[SYM[q][l]] * At[SYM[l][b]];
+                Asq += At[SYM[a][q]] * Aub;
+            }
+            const double t2 = alpha * (K * At[ab] - 2.0 * Asq);
+            double L = advect(F_AXX + ab, dAt[ab]);
+            for (int q = 0; q < 3; ++q) L += At[SYM[a][q]] * dbe[q][b] + At[SYM[b][q]] * dbe[q][a];
+            L -= (2.0 / 3.0) * At[ab] * divb;
+            raw[ab] = t1 + t2 + L;
+        }
+    double trr = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) trr += gu[SYM[a][b]] * raw[SYM[a][b]];
+    for (int ab = 0; ab < 6; ++ab) out[F_AXX + ab] = raw[ab] - (1.0 / 3.0) * gt[ab] * trr;
+
+    // Gamma-hat (ccz4.cpp:747-815)
+    for (int a = 0; a < 3; ++a) {
+        double T1 = 0.0;
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) T1 += chr[a][SYM[b][c]] * At[SYM[b][c]];
+        T1 *= 2.0 * alpha;
+        double T2 = 0.0;
+        for (int b = 0; b < 3; ++b) T2 += gu[SYM[a][b]] * dK[b];
+        T2 *= -(4.0 / 3.0) * alpha;
+        double T3 = 0.0;
+        for (int b = 0; b < 3; ++b) T3 += gu[SYM[a][b]] * Si[b];
+        T3 *= -16.0 * PI * alpha;
+        double T4 = 0.0;
+        for (int b = 0; b < 3; ++b) {
+            double Au = 0.0;
+            for (int c = 0; c < 3; ++c) Au += gu[SYM[a][c]] * At[SYM[c][b]];
+            T4 += Au * dchi[b];
+        }
+        T4 *= 3.0 * alpha / (chi + 1e-30);
+        double T5 = 0.0;
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) T5 += gu[SYM[b][c]] * ddbe[a][b][c];
+        double T6 = 0.0;
+        for (int b = 0; b < 3; ++b) {
+            double dd = 0.0;
+            for (int c = 0; c < 3; ++c) dd += ddbe[c][c][b];
+            T6 += gu[SYM[a][b]] * dd;
+        }
+        T6 *= (1.0 / 3.0);
+        double T7 = 0.0;
+        for (int b = 0; b < 3; ++b) T7 -= Gh[b] * dbe[a][b];
+        T7 += advect(F_GH + a, dGh[a]);
+        T7 += (2.0 / 3.0) * Gh[a] * divb;
+        const double T8 = -2.0 * P.kappa1 * alpha * Gh[a];
+        out[F_GH + a] = T1 + T2 + T3 + T4 + T5 + T6 + T7 + T8;
+    }
+
+    // lapse (ccz4.cpp:820) and shift (ccz4.cpp:826-831)
+    out[F_ALPHA] = -2.0 * alpha * K + advect(F_ALPHA, dal);
+    for (int d = 0; d < 3; ++d)
+        out[F_BETA + d] = 0.75 * Gh[d] - P.eta * be[d] + advect(F_BETA + d, dbe[d]);
+}
+
+// Kreiss-Oliger sum at one cell for all fields (ccz4.cpp:886-931)
+void cell_ko(const Geom& g, double sigma, const double* u, int i, int j, int k, double* diss) {
+    static const double wts[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+    Cell C(g, u, i, j, k);
+    for (int v = 0; v < NV; ++v) diss[v] = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double cf = sigma / (64.0 * g.dx[d]);
+        for (int s = 0; s < 7; ++s) {
+            const double w = cf * wts[s];
+            for (int v = 0; v < NV; ++v) diss[v] += w * C.at(v, d, s - 3);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost fills
+// ---------------------------------------------------------------------------
+double f_inf(int v) { return (v == 0 || v == 1 || v == 4 || v == 6 || v == 18) ? 1.0 : 0.0; }
+
+// 1/r fall-off written into every ghost layer, passes X then Y then Z, each over the
+// full transverse extent (main.cpp:358-484)
+void fill_sommerfeld(const Geom& g, double* u) {
+    const int ng = g.ng;
+    const int is = ng;
+    int ie[3];
+    for (int d = 0; d < 3; ++d) ie[d] = g.nt[d] - ng - 1;
+    for (int v = 0; v < NV; ++v) {
+        double* U = u + v * g.stride;
+        const double fi = f_inf(v);
+        auto rr = [](double a, double b, double c) { return std::sqrt(a * a + b * b + c * c) + 1e-30; };
+        for (int d = 0; d < 3; ++d) {
+            const int e1 = (d == 0) ? 1 : 0;
+            const int e2 = (d == 2) ? 1 : 2;
+            for (int q2 = 0; q2 < g.nt[e2]; ++q2)
+                for (int q1 = 0; q1 < g.nt[e1]; ++q1)
+                    for (int gi = 0; gi < ng; ++gi)
+                        for (int side = 0; side < 2; ++side) {
+                            const int ib = side == 0 ? is : ie[d];
+                            const int ig = side == 0 ? is - 1 - gi : ie[d] + 1 + gi;
+                            int Ib[3], Ig[3];
+                            Ib[d] = ib;
+                            Ig[d] = ig;
+                            Ib[e1] = Ig[e1] = q1;
+                            Ib[e2] = Ig[e2] = q2;
+                            const double r_int = rr(g.x(0, Ib[0]), g.x(1, Ib[1]), g.x(2, Ib[2]));
+                            const double r_gh = rr(g.x(0, Ig[0]), g.x(1, Ig[1]), g.x(2, Ig[2]));
+                            const double fint = U[g.at(Ib[0], Ib[1], Ib[2])];
+                            U[g.at(Ig[0], Ig[1], Ig[2])] = fi + (fint - fi) * r_int / r_gh;
+                        }
+        }
+    }
+}
+
+// zero-gradient copy of the nearest interior slab, X then Y then Z (amr.cpp:436-480)
+void fill_outflow(const Geom& g, double* u) {
+    const int ng = g.ng;
+    for (int v = 0; v < NV; ++v) {
+        double* U = u + v * g.stride;
+        for (int d = 0; d < 3; ++d) {
+            const int e1 = (d == 0) ? 1 : 0;
+            const int e2 = (d == 2) ? 1 : 2;
+            const int ie = ng + g.n[d];
+            for (int q2 = 0; q2 < g.nt[e2]; ++q2)
+                for (int q1 = 0; q1 < g.nt[e1]; ++q1)
+                    for (int gi = 0; gi < ng; ++gi) {
+                        int A[3], B[3], C2[3], D[3];
+                        A[e1] = B[e1] = C2[e1] = D[e1] = q1;
+                        A[e2] = B[e2] = C2[e2] = D[e2] = q2;
+                        A[d] = gi;
+                        B[d] = ng;
+                        C2[d] = ie + gi;
+                        D[d] = ie - 1;
+                        U[g.at(A[0], A[1], A[2])] = U[g.at(B[0], B[1], B[2])];
+                        U[g.at(C2[0], C2[1], C2[2])] = U[g.at(D[0], D[1], D[2])];
+                    }
+        }
+    }
+}
+
+// periodic wrap of every ghost cell (a BC mode the reference does not have; used for
+// the gauge-wave box). Ghost index i maps to ng + ((i - ng) mod n).
+void fill_periodic(const Geom& g, double* u) {
+    const int ng = g.ng;
+    auto wrap = [&](int d, int i) {
+        int r = (i - ng) % g.n[d];
+        if (r < 0) r += g.n[d];
+        return ng + r;
+    };
+    for (int v = 0; v < NV; ++v) {
+        double* U = u + v * g.stride;
+        for (int k = 0; k < g.nt[2]; ++k)
+            for (int j = 0; j < g.nt[1]; ++j)
+                for (int i = 0; i < g.nt[0]; ++i) {
+                    const bool inner = i >= ng && i < ng + g.n[0] && j >= ng && j < ng + g.n[1] && k >= ng &&
+                        k < ng + g.n[2];
+                    if (inner) continue;
+                    U[g.at(i, j, k)] = U[g.at(wrap(0, i), wrap(1, j), wrap(2, k))];
+                }
+    }
+}
+
+void apply_floors(const Geom& g, double* u) {
+    for (long p = 0; p < g.stride; ++p) {
+        double& c = u[F_CHI * g.stride + p];
+        if (!std::isfinite(c) || c < 1.0e-4) c = 1.0e-4;
+        if (c > 1.5) c = 1.5;
+        double& a = u[F_ALPHA * g.stride + p];
+        if (!std::isfinite(a) || a < 1.0e-6) a = 1.0e-6;
+    }
+}
+
+void apply_algebraic(const Geom& g, double* u) {
+    for (long p = 0; p < g.stride; ++p) {
+        double* G[6];
+        double* A[6];
+        for (int a = 0; a < 6; ++a) {
+            G[a] = &u[(F_GXX + a) * g.stride + p];
+            A[a] = &u[(F_AXX + a) * g.stride + p];
+        }
+        double &xx = *G[0], &xy = *G[1], &xz = *G[2], &yy = *G[3], &yz = *G[4], &zz = *G[5];
+        const double det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+        if (std::isfinite(det) && det > 1.0e-10) {
+            const double s = std::cbrt(1.0 / det);
+            for (int a = 0; a < 6; ++a) *G[a] *= s;
+        } else {
+            xx = 1.0; xy = 0.0; xz = 0.0; yy = 1.0; yz = 0.0; zz = 1.0;
+        }
+        const double cxx = yy * zz - yz * yz;
+        const double cxy = xz * yz - xy * zz;
+        const double cxz = xy * yz - xz * yy;
+        const double cyy = xx * zz - xz * xz;
+        const double cyz = xz * xy - xx * yz;
+        const double czz = xx * yy - xy * xy;
+        const double tr = cxx * *A[0] + cyy * *A[3] + czz * *A[5] + 2.0 * (cxy * *A[1] + cxz * *A[2] + cyz * *A[4]);
+        const double t3 = tr / 3.0;
+        for (int a = 0; a < 6; ++a) *A[a] -= t3 * *G[a];
+    }
+}
+
+template <class F>
+void for_interior(const Geom& g, F&& fn) {
+    const int ng = g.ng;
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int k = ng; k < ng + g.n[2]; ++k)
+        for (int j = ng; j < ng + g.n[1]; ++j)
+            for (int i = ng; i < ng + g.n[0]; ++i) fn(i, j, k);
+}
+
+void rhs_block(const Geom& g, const Params& P, const double* u, double* rhs, const double* rho, const double* Si,
+               const double* Sij, const double* S) {
+    for_interior(g, [&](int i, int j, int k) {
+        double o[NV];
+        cell_rhs(g, P, u, i, j, k, rho, Si, Sij, S, o);
+        const long c = g.at(i, j, k);
+        for (int v = 0; v < NV; ++v) rhs[v * g.stride + c] = o[v];
+    });
+    if (P.ko_sigma > 0.0) {
+        for_interior(g, [&](int i, int j, int k) {
+            double dd[NV];
+            cell_ko(g, P.ko_sigma, u, i, j, k, dd);
+            const long c = g.at(i, j, k);
+            for (int v = 0; v < NV; ++v) rhs[v * g.stride + c] += dd[v];
+        });
+    }
+}
+
+void combine(const Geom& g, double* dst, double a1, const double* s1, double a2, const double* s2, double a3,
+             double dt, const double* r) {
+    for (int v = 0; v < NV; ++v) {
+        const long o = v * g.stride;
+        for_interior(g, [&](int i, int j, int k) {
+            const long c = o + g.at(i, j, k);
+            dst[c] = a1 * s1[c] + a2 * s2[c] + a3 * dt * r[c];
+        });
+    }
+}
+
+void fill_bc(const Geom& g, double* u, int mode) {
+    if (mode == 0)
+        fill_sommerfeld(g, u);
+    else if (mode == 1)
+        fill_periodic(g, u);
+    else if (mode == 2)
+        fill_outflow(g, u);
+}
+
+} // namespace
+
+// ===========================================================================
+// C entry points (test infrastructure)
+// ===========================================================================
+extern "C" {
+
+// geometry is passed as n[3], ng, lo[3], dx[3]; params as
+// {kappa1, kappa2, eta, ko_sigma, chi_blend_center, chi_blend_width}
+
+// Pinned by: compiled reference (oracle/_ref) on BL / noisy BL / gauge-wave blocks,
+// committed golden fixtures, flat-space zero, vacuum == zero sources.
+void orc_rhs(const int* n, int ng, const double* lo, const double* dx, const double* prm, const double* u,
+             double* rhs, const double* rho, const double* Si, const double* Sij, const double* S) {
+    Geom g = make_geom(n, ng, lo, dx);
+    Params P{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5]};
+    rhs_block(g, P, u, rhs, rho, Si, Sij, S);
+}
+
+// Pinned by: compiled reference, polynomial null space (degree <= 5), alternating mode.
+void orc_ko(const int* n, int ng, const double* lo, const double* dx, double sigma, const double* u, double* rhs) {
+    Geom g = make_geom(n, ng, lo, dx);
+    if (sigma <= 0.0) return;
+    for_interior(g, [&](int i, int j, int k) {
+        double dd[NV];
+        cell_ko(g, sigma, u, i, j, k, dd);
+        const long c = g.at(i, j, k);
+        for (int v = 0; v < NV; ++v) rhs[v * g.stride + c] += dd[v];
+    });
+}
+
+// Pinned by: compiled reference, flat-space zero.
+// ham: stride doubles; mom: 3*stride doubles (AoS); ghosts set to zero.
+void orc_constraints(const int* n, int ng, const double* lo, const double* dx, const double* u, double* ham,
+                     double* mom) {
+    Geom g = make_geom(n, ng, lo, dx);
+    std::fill(ham, ham + g.stride, 0.0);
+    std::fill(mom, mom + 3 * g.stride, 0.0);
+    for_interior(g, [&](int i, int j, int k) {
+        Cell C(g, u, i, j, k);
+        const double chi = C.f(F_CHI, 0);
+        const double K = C.f(F_K, 0);
+        double gt[6], At[6], Gh[3];
+        for (int a = 0; a < 6; ++a) {
+            gt[a] = C.f(F_GXX + a, 0);
+            At[a] = C.f(F_AXX + a, 0);
+        }
+        for (int a = 0; a < 3; ++a) Gh[a] = C.f(F_GH + a, 0);
+        const double det = gt[0] * (gt[3] * gt[5] - gt[4] * gt[4]) - gt[1] * (gt[1] * gt[5] - gt[4] * gt[2]) +
+            gt[2] * (gt[1] * gt[4] - gt[3] * gt[2]);
+        const double idet = 1.0 / (det + 1.0e-30);
+        const double gu[6] = {(gt[3] * gt[5] - gt[4] * gt[4]) * idet, (gt[2] * gt[4] - gt[1] * gt[5]) * idet,
+                              (gt[1] * gt[4] - gt[2] * gt[3]) * idet, (gt[0] * gt[5] - gt[2] * gt[2]) * idet,
+                              (gt[1] * gt[2] - gt[0] * gt[4]) * idet, (gt[0] * gt[3] - gt[1] * gt[1]) * idet};
+        double AA = 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c)
+                    for (int d = 0; d < 3; ++d) AA += gu[SYM[a][c]] * gu[SYM[b][d]] * At[SYM[a][b]] * At[SYM[c][d]];
+        double dgt[6][3], ddgt[6][3][3], dchi[3], ddchi[3][3], dGh[3][3], dAt[6][3], dK[3];
+        for (int v = 0; v < 6; ++v)
+            for (int d = 0; d < 3; ++d) {
+                dgt[v][d] = C.d1(F_GXX + v, d);
+                for (int e = d; e < 3; ++e) ddgt[v][d][e] = ddgt[v][e][d] = C.d2(F_GXX + v, d, e);
+            }
+        for (int d = 0; d < 3; ++d) {
+            dchi[d] = C.d1(F_CHI, d);
+            for (int e = d; e < 3; ++e) ddchi[d][e] = ddchi[e][d] = C.d2(F_CHI, d, e);
+        }
+        // note the index order of this Christoffel sum differs from the RHS one
+        // (ccz4.cpp:1035-1047); it is kept so the monitor agrees to round-off
+        double chr[3][6];
+        for (int q = 0; q < 3; ++q)
+            for (int a = 0; a < 3; ++a)
+                for (int b = a; b < 3; ++b) {
+                    double s = 0.0;
+                    for (int m = 0; m < 3; ++m)
+                        s += 0.5 * gu[SYM[q][m]] * (dgt[SYM[m][a]][b] + dgt[SYM[m][b]][a] - dgt[SYM[a][b]][m]);
+                    chr[q][SYM[a][b]] = s;
+                }
+        for (int d = 0; d < 3; ++d)
+            for (int a = 0; a < 3; ++a) dGh[a][d] = C.d1(F_GH + a, d);
+        double Rt[6], Rc[6];
+        for (int a = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b) {
+                const int ab = SYM[a][b];
+                double R = 0.0;
+                for (int l = 0; l < 3; ++l)
+                    for (int m = 0; m < 3; ++m) R += -0.5 * gu[SYM[l][m]] * ddgt[ab][l][m];
+                for (int q = 0; q < 3; ++q) {
+                    R += 0.5 * (gt[SYM[q][a]] * dGh[q][b] + gt[SYM[q][b]] * dGh[q][a]);
+                    double G = 0.0;
+                    for (int m = 0; m < 3; ++m) G += gt[SYM[q][m]] * chr[m][ab];
+                    R += Gh[q] * G;
+                }
+                for (int l = 0; l < 3; ++l)
+                    for (int m = 0; m < 3; ++m) {
+                        const double glm = gu[SYM[l][m]];
+                        for (int q = 0; q < 3; ++q) {
+                            double Gb = 0.0, Ga = 0.0, Gab = 0.0;
+                            for (int n2 = 0; n2 < 3; ++n2) {
+                                Gb += gt[SYM[q][n2]] * chr[n2][SYM[b][m]];
+                                Ga += gt[SYM[q][n2]] * chr[n2][SYM[a][m]];
+                                Gab += gt[SYM[q][n2]] * chr[n2][ab];
+                            }
+                            R += glm * (chr[q][SYM[l][a]] * Gb + chr[q][SYM[l][b]] * Ga);
+                            R -= glm * chr[q][SYM[l][m]] * Gab;
+                        }
+                    }
+                Rt[ab] = R;
+                double Dab = ddchi[a][b];
+                for (int q = 0; q < 3; ++q) Dab -= chr[q][ab] * dchi[q];
+                double lapchi = 0.0, gradchi2 = 0.0;
+                for (int l = 0; l < 3; ++l)
+                    for (int m = 0; m < 3; ++m) {
+                        const double glm = gu[SYM[l][m]];
+                        double Dlm = ddchi[l][m];
+                        for (int q = 0; q < 3; ++q) Dlm -= chr[q][SYM[l][m]] * dchi[q];
+                        lapchi += glm * Dlm;
+                        gradchi2 += glm * dchi[l] * dchi[m];
+                    }
+                Rc[ab] = (0.5 / (chi + 1e-30)) * (Dab + gt[ab] * lapchi) -
+                    (0.25 / ((chi * chi) + 1e-30)) * (dchi[a] * dchi[b] - 3.0 * gt[ab] * gradchi2);
+            }
+        double Rs = 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Rs += gu[SYM[a][b]] * (Rt[SYM[a][b]] + Rc[SYM[a][b]]);
+        Rs *= chi;
+        const double rho = 0.0;
+        const long c = C.c;
+        ham[c] = Rs + (2.0 / 3.0) * K * K - AA - 16.0 * PI * rho;
+        for (int v = 0; v < 6; ++v)
+            for (int d = 0; d < 3; ++d) dAt[v][d] = C.d1(F_AXX + v, d);
+        for (int d = 0; d < 3; ++d) dK[d] = C.d1(F_K, d);
+        for (int a = 0; a < 3; ++a) {
+            double M = 0.0;
+            for (int b = 0; b < 3; ++b)
+                for (int q = 0; q < 3; ++q) {
+                    const double gbq = gu[SYM[b][q]];
+                    double DA = dAt[SYM[q][a]][b];
+                    for (int m = 0; m < 3; ++m) DA -= chr[m][SYM[b][q]] * At[SYM[m][a]] + chr[m][SYM[b][a]] * At[SYM[q][m]];
+                    M += gbq * DA;
+                    M -= (3.0 / (2.0 * chi + 1e-30)) * At[SYM[a][b]] * gbq * dchi[q];
+                }
+            M -= (2.0 / 3.0) * dK[a];
+            mom[3 * c + a] = M;
+        }
+    });
+}
+
+// Pinned by: hand-made cells (non-finite, below/above limits) in tests.
+void orc_floors(const int* n, int ng, const double* lo, const double* dx, double* u) {
+    apply_floors(make_geom(n, ng, lo, dx), u);
+}
+
+// Pinned by: det == 1 and trace-free results, degenerate-metric reset, hand-made cells.
+void orc_algebraic(const int* n, int ng, const double* lo, const double* dx, double* u) {
+    apply_algebraic(make_geom(n, ng, lo, dx), u);
+}
+
+// mode 0 = Sommerfeld 1/r fall-off, 1 = periodic, 2 = outflow copy.
+// Pinned by: closed-form 1/r profiles (exact fixed point), constant fields, per-pass
+// corner semantics checked against a literal three-pass evaluation in tests.
+void orc_fill_ghosts(const int* n, int ng, const double* lo, const double* dx, double* u, int mode) {
+    fill_bc(make_geom(n, ng, lo, dx), u, mode);
+}
+
+// d = a1*s1 + a2*s2 + a3*dt*r on interior cells
+void orc_combine(const int* n, int ng, const double* lo, const double* dx, double* dst, double a1, const double* s1,
+                 double a2, const double* s2, double a3, double dt, const double* r) {
+    combine(make_geom(n, ng, lo, dx), dst, a1, s1, a2, s2, a3, dt, r);
+}
+
+// One SSP-RK3 step in the reference order (main.cpp:673-689), vacuum, one block.
+// stage and rhs are caller scratch of the same shape as u. bc_mode as orc_fill_ghosts.
+// Pinned by: the pieces above, plus the exact SSP-RK3 amplification on a linear ODE
+// (flat space with beta = Gamma-hat = 0 reduces the shift equation to d beta/dt = -eta beta).
+void orc_rk3_step(const int* n, int ng, const double* lo, const double* dx, const double* prm, double* u,
+                  double* stage, double* rhs, double dt, int bc_mode) {
+    Geom g = make_geom(n, ng, lo, dx);
+    Params P{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5]};
+    rhs_block(g, P, u, rhs, nullptr, nullptr, nullptr, nullptr);
+    combine(g, stage, 1.0, u, 0.0, u, 1.0, dt, rhs);
+    apply_floors(g, stage);
+    fill_bc(g, stage, bc_mode);
+    rhs_block(g, P, stage, rhs, nullptr, nullptr, nullptr, nullptr);
+    combine(g, stage, 0.75, u, 0.25, stage, 0.25, dt, rhs);
+    apply_floors(g, stage);
+    fill_bc(g, stage, bc_mode);
+    rhs_block(g, P, stage, rhs, nullptr, nullptr, nullptr, nullptr);
+    combine(g, u, 1.0 / 3.0, u, 2.0 / 3.0, stage, 2.0 / 3.0, dt, rhs);
+    apply_floors(g, u);
+    apply_algebraic(g, u);
+    fill_bc(g, u, bc_mode);
+}
+
+// Initial data ---------------------------------------------------------------
+
+// chi = 1, conformal metric = identity, lapse = 1, everything else 0 (ccz4.cpp:1165-1188)
+void orc_init_flat(const int* n, int ng, const double* lo, const double* dx, double* u) {
+    Geom g = make_geom(n, ng, lo, dx);
+    std::fill(u, u + NV * g.stride, 0.0);
+    for (long p = 0; p < g.stride; ++p) {
+        u[F_CHI * g.stride + p] = 1.0;
+        u[(F_GXX + 0) * g.stride + p] = 1.0;
+        u[(F_GXX + 3) * g.stride + p] = 1.0;
+        u[(F_GXX + 5) * g.stride + p] = 1.0;
+        u[F_ALPHA * g.stride + p] = 1.0;
+    }
+}
+
+// Pinned by: psi(1) = 1.5, psi(0.5) = 2 and the two-hole values of the reference tests,
+// and by the compiled reference's BrillLindquist::apply.
+void orc_init_bl(const int* n, int ng, const double* lo, const double* dx, int nbh, const double* mass,
+                 const double* pos, double* u) {
+    Geom g = make_geom(n, ng, lo, dx);
+    orc_init_flat(n, ng, lo, dx, u);
+    for (int k = 0; k < g.nt[2]; ++k)
+        for (int j = 0; j < g.nt[1]; ++j)
+            for (int i = 0; i < g.nt[0]; ++i) {
+                const double x = g.x(0, i), y = g.x(1, j), z = g.x(2, k);
+                double psi = 1.0;
+                for (int b = 0; b < nbh; ++b) {
+                    const double ax = x - pos[3 * b], ay = y - pos[3 * b + 1], az = z - pos[3 * b + 2];
+                    const double r = std::sqrt(ax * ax + ay * ay + az * az + 1.0e-30);
+                    psi += mass[b] / (2.0 * r);
+                }
+                const double p2 = psi * psi;
+                const double chi = 1.0 / (p2 * p2);
+                const long c = g.at(i, j, k);
+                u[F_CHI * g.stride + c] = chi;
+                u[F_ALPHA * g.stride + c] = std::sqrt(chi);
+                double dp[3] = {0.0, 0.0, 0.0};
+                for (int b = 0; b < nbh; ++b) {
+                    const double rx = x - pos[3 * b], ry = y - pos[3 * b + 1], rz = z - pos[3 * b + 2];
+                    const double r2 = rx * rx + ry * ry + rz * rz + 1.0e-20;
+                    const double r3 = r2 * std::sqrt(r2);
+                    const double cc = -mass[b] / (2.0 * r3);
+                    dp[0] += cc * rx;
+                    dp[1] += cc * ry;
+                    dp[2] += cc * rz;
+                }
+                const double gf = 8.0 / (3.0 * psi);
+                for (int a = 0; a < 3; ++a) u[(F_GH + a) * g.stride + c] = gf * dp[a];
+            }
+}
+
+// Pinned by: alpha(L/4) = sqrt(1+A) and chi(x) closed form (reference gauge-wave tests).
+void orc_init_gauge_wave(const int* n, int ng, const double* lo, const double* dx, double A, double L, double* u) {
+    Geom g = make_geom(n, ng, lo, dx);
+    std::fill(u, u + NV * g.stride, 0.0);
+    for (int k = 0; k < g.nt[2]; ++k)
+        for (int j = 0; j < g.nt[1]; ++j)
+            for (int i = 0; i < g.nt[0]; ++i) {
+                const double x = g.x(0, i);
+                const double H = A * std::sin(2.0 * PI * x / L);
+                const double detg = 1.0 + H;
+                const double chi = std::pow(detg, -1.0 / 3.0);
+                const long c = g.at(i, j, k);
+                u[F_CHI * g.stride + c] = chi;
+                u[(F_GXX + 0) * g.stride + c] = (1.0 + H) * chi;
+                u[(F_GXX + 3) * g.stride + c] = chi;
+                u[(F_GXX + 5) * g.stride + c] = chi;
+                u[F_ALPHA * g.stride + c] = std::sqrt(1.0 + H);
+            }
+}
+
+// Portable counter-based Gaussian noise: splitmix64 outputs at counters 2q+1 and 2q+2
+// of a stream seeded with `seed`, Box-Muller (cosine branch). q = v*stride + flat.
+// Added to every cell (ghosts included) of each field whose bit is set in `mask`.
+static uint64_t sm64(uint64_t s) {
+    s += 0x9E3779B97F4A7C15ull;
+    s = (s ^ (s >> 30)) * 0xBF58476D1CE4E5B9ull;
+    s = (s ^ (s >> 27)) * 0x94D049BB133111EBull;
+    return s ^ (s >> 31);
+}
+double orc_gauss(uint64_t seed, uint64_t q) {
+    const uint64_t z1 = sm64(seed + (2 * q) * 0x9E3779B97F4A7C15ull);
+    const uint64_t z2 = sm64(seed + (2 * q + 1) * 0x9E3779B97F4A7C15ull);
+    const double u1 = ((double)(z1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(z2 >> 11) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * PI * u2);
+}
+void orc_add_noise(const int* n, int ng, const double* lo, const double* dx, double* u, uint64_t seed, double amp,
+                   unsigned mask) {
+    Geom g = make_geom(n, ng, lo, dx);
+    for (int v = 0; v < NV; ++v) {
+        if (!((mask >> v) & 1u)) continue;
+        for (long p = 0; p < g.stride; ++p) u[v * g.stride + p] += amp * orc_gauss(seed, (uint64_t)v * g.stride + p);
+    }
+}
+
+// AMR transfer -----------------------------------------------------------------
+
+// Trilinear coarse -> fine into every fine cell, ghosts included (amr.cpp:484-564).
+// Pinned by: exactness on constant and linear fields away from clamps; compiled reference.
+void orc_prolongate(const int* cn, int cng, const double* clo, const double* cdx, const double* cu, const int* fn,
+                    int fng, const double* flo, const double* fdx, double* fu, int nvars) {
+    Geom C = make_geom(cn, cng, clo, cdx), F = make_geom(fn, fng, flo, fdx);
+    for (int v = 0; v < nvars; ++v)
+        for (int k = 0; k < F.nt[2]; ++k)
+            for (int j = 0; j < F.nt[1]; ++j)
+                for (int i = 0; i < F.nt[0]; ++i) {
+                    const double X[3] = {F.x(0, i), F.x(1, j), F.x(2, k)};
+                    double cf[3], fr[3], ax[3];
+                    int c0[3], c1[3];
+                    for (int d = 0; d < 3; ++d) {
+                        cf[d] = (X[d] - C.x(d, C.ng)) / C.dx[d] + C.ng;
+                        int ci = (int)std::round(cf[d]);
+                        ci = std::max(0, std::min(ci, C.nt[d] - 1));
+                        fr[d] = cf[d] - (double)ci;
+                        int cj = std::min(ci + (fr[d] >= 0.0 ? 1 : -1), C.nt[d] - 1);
+                        cj = std::max(cj, 0);
+                        c0[d] = ci;
+                        c1[d] = cj;
+                        ax[d] = std::abs(fr[d]);
+                    }
+                    const double* U = cu + v * C.stride;
+                    auto at = [&](int a, int b, int c) { return U[C.at(a, b, c)]; };
+                    const double w000 = (1 - ax[0]) * (1 - ax[1]) * (1 - ax[2]);
+                    const double w100 = ax[0] * (1 - ax[1]) * (1 - ax[2]);
+                    const double w010 = (1 - ax[0]) * ax[1] * (1 - ax[2]);
+                    const double w001 = (1 - ax[0]) * (1 - ax[1]) * ax[2];
+                    const double w110 = ax[0] * ax[1] * (1 - ax[2]);
+                    const double w101 = ax[0] * (1 - ax[1]) * ax[2];
+                    const double w011 = (1 - ax[0]) * ax[1] * ax[2];
+                    const double w111 = ax[0] * ax[1] * ax[2];
+                    fu[v * F.stride + F.at(i, j, k)] = w000 * at(c0[0], c0[1], c0[2]) + w100 * at(c1[0], c0[1], c0[2]) +
+                        w010 * at(c0[0], c1[1], c0[2]) + w001 * at(c0[0], c0[1], c1[2]) +
+                        w110 * at(c1[0], c1[1], c0[2]) + w101 * at(c1[0], c0[1], c1[2]) +
+                        w011 * at(c0[0], c1[1], c1[2]) + w111 * at(c1[0], c1[1], c1[2]);
+                }
+}
+
+// 2x2x2 average fine -> covered coarse interior cells (amr.cpp:566-626), ratio 2.
+void orc_restrict(const int* fn, int fng, const double* flo, const double* fdx, const double* fu, const int* cn,
+                  int cng, const double* clo, const double* cdx, double* cu, int nvars) {
+    Geom F = make_geom(fn, fng, flo, fdx), C = make_geom(cn, cng, clo, cdx);
+    const int ratio = 2;
+    for (int v = 0; v < nvars; ++v)
+        for (int ck = C.ng; ck < C.ng + C.n[2]; ++ck)
+            for (int cj = C.ng; cj < C.ng + C.n[1]; ++cj)
+                for (int ci = C.ng; ci < C.ng + C.n[0]; ++ci) {
+                    const double X[3] = {C.x(0, ci), C.x(1, cj), C.x(2, ck)};
+                    int b[3];
+                    bool ok = true;
+                    for (int d = 0; d < 3; ++d) {
+                        b[d] = (int)std::round((X[d] - F.x(d, 0)) / F.dx[d]) - ratio / 2;
+                        ok = ok && b[d] >= F.ng && b[d] + ratio <= F.ng + F.n[d] + F.ng;
+                    }
+                    if (!ok) continue;
+                    double s = 0.0;
+                    for (int rk = 0; rk < ratio; ++rk)
+                        for (int rj = 0; rj < ratio; ++rj)
+                            for (int ri = 0; ri < ratio; ++ri) s += fu[v * F.stride + F.at(b[0] + ri, b[1] + rj, b[2] + rk)];
+                    cu[v * C.stride + C.at(ci, cj, ck)] = s / (ratio * ratio * ratio);
+                }
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+} // extern "C"

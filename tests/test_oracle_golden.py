@@ -1,0 +1,64 @@
+"""The CPU oracle against golden vectors written by the compiled GRANITE reference.
+
+The fixtures (tests/golden/*.npz, made by tests/golden/make_golden.py from the reference
+sources) hold the reference's computeRHS / applyDissipation / computeConstraints outputs
+and its initial data. The oracle restates those routines with the same operation order,
+so agreement is exact (bitwise)."""
+import os
+
+import numpy as np
+import pytest
+
+from golden_cases import NAMES, load, make_input, sha, sources
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_inputs_regenerate_bitwise(oracle, name):
+    rec, z = load(name)
+    g, u = make_input(oracle, rec)
+    assert sha(u) == str(z["input_sha256"])
+    # the oracle's initial data equals the reference's BrillLindquist::apply / setGaugeWaveData
+    if rec["kind"] == "bl":
+        u0 = oracle.init_bl(g, rec["masses"], rec["positions"])
+    else:
+        u0 = oracle.init_gauge_wave(g, rec["A"], rec["L"])
+    assert np.array_equal(u0, z["init"])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_rhs_matches_reference_bitwise(oracle, name):
+    rec, z = load(name)
+    g, u = make_input(oracle, rec)
+    I = g.interior()
+    assert np.array_equal(oracle.rhs(g, u)[I], z["rhs"])
+    if "rhs_nodiss" in z:
+        assert np.array_equal(oracle.rhs(g, u, ko_sigma=0.0)[I], z["rhs_nodiss"])
+    if "ko" in z:
+        assert np.array_equal(oracle.ko(g, u, g.zeros(), 0.1)[I], z["ko"])
+    if "rhs_sources" in z:
+        assert np.array_equal(oracle.rhs(g, u, sources=sources(g, 99))[I], z["rhs_sources"])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_constraints_match_reference_bitwise(oracle, name):
+    rec, z = load(name)
+    g, u = make_input(oracle, rec)
+    I = g.interior()
+    h, m = oracle.constraints(g, u)
+    assert np.array_equal(h[I[1:]], z["ham"])
+    assert np.array_equal(m[I[1:]], z["mom"])
+
+
+def test_amr_transfer_matches_reference(oracle):
+    from oracle.oracle import Geom
+
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "amr_transfer.npz"), allow_pickle=False)
+    gc = Geom((8, 8, 8), 4, (-2.0, -2.0, -2.0), (2.0, 2.0, 2.0))
+    gf = Geom((8, 8, 8), 4, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    uc = oracle.add_noise(gc, oracle.init_bl(gc, [1.0], [[0.1, -0.05, 0.02]]), 3, 1e-2, list(range(22)))
+    uf = oracle.add_noise(gf, oracle.init_bl(gf, [1.0], [[0.1, -0.05, 0.02]]), 4, 1e-2, list(range(22)))
+    assert sha(uc) == str(z["uc_sha256"]) and sha(uf) == str(z["uf_sha256"])
+    prol = oracle.prolongate(gc, uc, gf, np.zeros_like(uf))
+    rest = oracle.restrict(gf, uf, gc, uc.copy())
+    assert np.array_equal(prol[:4], z["prolongated"])
+    assert np.array_equal(rest[:4], z["restricted"])
