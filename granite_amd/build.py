@@ -31,17 +31,18 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source into LIB; always recompiles."""
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), *SOURCES, "-o", tmp]
+def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
+    """Compile every source into `out` (default: the in-tree library); always recompiles.
+    `defines` are extra -D macros (tile shape and prefetch depth of the stage kernel)."""
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *FLAGS, *["-D" + d for d in defines], "-I", os.path.join(ROOT, "include"), *SOURCES, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
-    out = subprocess.run(cmd, capture_output=True, text=True)
-    if out.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + out.stdout + out.stderr)
-    os.replace(tmp, LIB)
-    return LIB
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -76,7 +76,8 @@ _lib = None
 
 
 def library_path() -> str:
-    return os.path.join(_HERE, "libgranite_ccz4.so")
+    """In-tree library; GRANITE_CCZ4_LIB may name another build of the same C ABI."""
+    return os.environ.get("GRANITE_CCZ4_LIB") or os.path.join(_HERE, "libgranite_ccz4.so")
 
 
 def library():

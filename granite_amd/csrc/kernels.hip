@@ -11,6 +11,26 @@
 
 #include <cstdint>
 
+// tile shape and prefetch depth of the FP64 stage kernel (overridable at build time)
+#ifndef GR_FP64_TILE_X
+#define GR_FP64_TILE_X 8
+#endif
+#ifndef GR_FP64_TILE_Y
+#define GR_FP64_TILE_Y 8
+#endif
+#ifndef GR_FP64_TILE_Z
+#define GR_FP64_TILE_Z 4
+#endif
+#ifndef GR_FP64_PREFETCH
+#define GR_FP64_PREFETCH 1
+#endif
+#ifndef GR_FP64_KO_EXACT
+#define GR_FP64_KO_EXACT 1
+#endif
+#ifndef GR_FP64_MIN_WAVES
+#define GR_FP64_MIN_WAVES 1
+#endif
+
 namespace gr {
 
 namespace {
@@ -97,15 +117,18 @@ struct TileGeom {
 //
 // Every thread owns the same NL entries of the tile's halo box for all fields, so the
 // LDS slot and the global offset of each entry are computed once. Field loads are issued
-// PF fields ahead of their use (in the fixed FIELD_ORDER), so that their HBM latency
-// overlaps the evaluation of the fields in between; the pointwise u^n value needed by the
-// RK3 combine of a field travels with that field's prefetch.
+// PF fields ahead of their use (in the fixed FIELD_ORDER), and the LDS tile is double
+// buffered: while field p is evaluated from one buffer, field p+1 is written into the
+// other, so each field costs one barrier and its HBM latency overlaps the work on the
+// fields before it. The pointwise u^n value needed by the RK3 combine of a field travels
+// with that field's prefetch.
 template <int TX, int TY, int TZ, int MODE, int PF>
 struct TileIO {
     using T = TileGeom<TX, TY, TZ>;
     static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
+    static constexpr int LBUF = T::NLDS + 1; // one LDS buffer incl. a discard slot
 
-    double* lds;
+    double* lds; // 2 * LBUF doubles
     const double* in;
     const double* un;
     double* out;
@@ -133,13 +156,11 @@ struct TileIO {
                     b = k;
                     q = p - T::start(k);
                 }
-            int x, y, z;
-            // decode q inside box b (boxes are small constant shapes)
             const int nx = T::box[b][1] - T::box[b][0];
             const int ny = T::box[b][3] - T::box[b][2];
-            x = T::box[b][0] + q % nx;
-            y = T::box[b][2] + (q / nx) % ny;
-            z = T::box[b][4] + q / (nx * ny);
+            const int x = T::box[b][0] + q % nx;
+            const int y = T::box[b][2] + (q / nx) % ny;
+            const int z = T::box[b][4] + q / (nx * ny);
             const int gx = ox + x, gy = oy + y, gz = oz + z;
             const bool ok = p < T::TOTAL && gx < nt0 && gy < nt1 && gz < nt2;
             loff[i] = ok ? T::lds(x, y, z) : T::NLDS;
@@ -147,9 +168,12 @@ struct TileIO {
         }
 #pragma unroll
         for (int k = 0; k < PF; ++k) prefetch(k);
+        stage_to_lds(0);
+        prefetch(PF);
     }
 
     __device__ __forceinline__ void prefetch(int pos) {
+        if (pos >= NV) return;
         const int v = FIELD_ORDER[pos];
         const int slot = pos % PF;
         const double* f = in + v * stride;
@@ -158,19 +182,25 @@ struct TileIO {
         if (MODE != MODE_RHS) unb[slot] = un[v * stride + cidx];
     }
 
+    // writes the prefetched field at position pos into its LDS buffer
+    __device__ __forceinline__ void stage_to_lds(int pos) {
+        if (pos >= NV) return;
+        const int slot = pos % PF;
+        double* L = lds + (pos & 1) * LBUF;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) L[loff[i]] = buf[slot][i];
+        if (MODE != MODE_RHS) unv[FIELD_ORDER[pos]] = unb[slot];
+    }
+
     // ---- Stager
     __device__ double centre(int v) const { return in[v * stride + cidx]; }
     __device__ FixedStrides<1, T::BX, T::BX * T::BY> strides() const { return {}; }
     __device__ const double* field(int v) {
         const int pos = field_pos(v);
-        const int slot = pos % PF;
-        __syncthreads(); // every thread is done with the previous field's tile
-#pragma unroll
-        for (int i = 0; i < NL; ++i) lds[loff[i]] = buf[slot][i];
-        if (MODE != MODE_RHS) unv[v] = unb[slot];
-        __syncthreads();
-        if (pos + PF < NV) prefetch(pos + PF);
-        return lds + cl;
+        __syncthreads(); // field pos is in its buffer; every thread is done with field pos-1
+        stage_to_lds(pos + 1);
+        prefetch(pos + 1 + PF);
+        return lds + (pos & 1) * LBUF + cl;
     }
 
     // ---- Sink
@@ -234,9 +264,9 @@ __device__ __forceinline__ int tile_of_block(int bid, int ntiles, int remap) {
 }
 
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
-__global__ __launch_bounds__(TX* TY* TZ) void stage_kernel(StageArgs A) {
+__global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
     using T = TileGeom<TX, TY, TZ>;
-    __shared__ double lds[T::NLDS + 1];
+    __shared__ double lds[2 * (T::NLDS + 1)];
 
     const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
     const int tix = tile % A.tiles[0];
@@ -249,7 +279,7 @@ __global__ __launch_bounds__(TX* TY* TZ) void stage_kernel(StageArgs A) {
     const int gi = A.ng + (active ? ix : 0), gj = A.ng + (active ? iy : 0), gk = A.ng + (active ? iz : 0);
     const long cidx = gi + (long)A.nt[0] * (gj + (long)A.nt[1] * gk);
 
-    TileIO<TX, TY, TZ, MODE, 2> io;
+    TileIO<TX, TY, TZ, MODE, GR_FP64_PREFETCH> io;
     io.lds = lds;
     io.in = A.in;
     io.un = A.un;
@@ -274,11 +304,11 @@ __global__ __launch_bounds__(TX* TY* TZ) void stage_kernel(StageArgs A) {
     ccz4_cell<HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
 }
 
-constexpr int FP64_TX = 8, FP64_TY = 8, FP64_TZ = 4;
+constexpr int FP64_TX = GR_FP64_TILE_X, FP64_TY = GR_FP64_TILE_Y, FP64_TZ = GR_FP64_TILE_Z;
 
 template <int MODE, bool HAS_SRC>
 hipError_t launch_stage_t(const StageArgs& A, int nblk, hipStream_t s) {
-    stage_kernel<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, true>
+    stage_kernel<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0>
         <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
     return hipGetLastError();
 }
@@ -706,7 +736,10 @@ hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double*
     A.tiles[1] = (b.n[1] + FP64_TY - 1) / FP64_TY;
     A.tiles[2] = (b.n[2] + FP64_TZ - 1) / FP64_TZ;
     A.ntiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-    A.xcd_remap = (A.ntiles % 8 == 0) ? 1 : 0;
+#ifndef GR_XCD_REMAP
+#define GR_XCD_REMAP 1
+#endif
+    A.xcd_remap = (GR_XCD_REMAP && A.ntiles % 8 == 0) ? 1 : 0;
     A.K = make_stencil(b, ko_sigma);
     A.P = P;
     A.cf = cf;
