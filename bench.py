@@ -62,42 +62,59 @@ def main():
     import granite_amd as ga
 
     eng = ga.Engine(0)
-    blk = ga.GridBlock(0, 0, (N, N, N), (-HALF,) * 3, (HALF,) * 3, 4)
-    lvl = ga.Level([blk], eng)
-    lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
     p = ga.CCZ4Params(kappa1=0.02, kappa2=0.0, eta=2.0, ko_sigma=0.1)
-    step = dict(precision=ga.PREC_FP64, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
-
-    for _ in range(args.warmup):
-        lvl.rk3_step(p, DT, use_graph=True, **step)
-    lvl.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        lvl.rk3_step(p, DT, use_graph=True, **step)
-    lvl.sync()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
     cells = N ** 3
-    value = cells / (ms_per_step * 1e-3)
 
-    # per-kernel device times (HIP events on the engine stream, eager launches)
-    lvl.profile(True)
-    for k in range(3):
-        lvl.profile_read(k)
-    for _ in range(args.profile_steps):
-        lvl.rk3_step(p, DT, use_graph=False, **step)
-    t_stage, n_stage = lvl.profile_read(0)
-    t_final, n_final = lvl.profile_read(1)
-    t_ghost, n_ghost = lvl.profile_read(2)
-    lvl.profile(False)
-    nonfinite = lvl.nonfinite()
-    sumsq, count = lvl.constraints_l2()
+    def run_form(precision):
+        blk = ga.GridBlock(0, 0, (N, N, N), (-HALF,) * 3, (HALF,) * 3, 4)
+        lvl = ga.Level([blk], eng)
+        lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
+        step = dict(precision=precision, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
+        for _ in range(args.warmup):
+            lvl.rk3_step(p, DT, use_graph=True, **step)
+        lvl.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            lvl.rk3_step(p, DT, use_graph=True, **step)
+        lvl.sync()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3 / args.steps
+        # per-kernel device times (HIP events on the engine stream, eager launches)
+        lvl.profile(True)
+        for k in range(3):
+            lvl.profile_read(k)
+        for _ in range(args.profile_steps):
+            lvl.rk3_step(p, DT, use_graph=False, **step)
+        prof = [lvl.profile_read(k) for k in range(3)]
+        lvl.profile(False)
+        r = {
+            "ms_per_step": ms,
+            "value": cells / (ms * 1e-3),
+            "stage_avg_ms": prof[0][0] / max(prof[0][1], 1),
+            "stage_launches": prof[0][1],
+            "final_avg_ms": prof[1][0] / max(prof[1][1], 1),
+            "final_launches": prof[1][1],
+            "ghost_avg_ms": prof[2][0] / max(prof[2][1], 1),
+            "nonfinite": lvl.nonfinite(),
+            "t": (args.warmup + args.steps + args.profile_steps) * DT,
+        }
+        sumsq, count = lvl.constraints_l2()
+        r["ham_l2"] = (sumsq / count) ** 0.5 if count else None
+        lvl.close()
+        return r
+
+    f64 = run_form(ga.PREC_FP64)
+    mix = run_form(ga.PREC_MIXED)
+    value = f64["value"]
+    ms_per_step = f64["ms_per_step"]
 
     # stage kernels of RK3 stages 1 and 2 alternate: stage 1 reads u (stencil) and writes
     # u1 = 352 B/cell; stage 2 reads u1 (stencil) and u^n and writes u2 = 528 B/cell
-    stage_avg_ms = t_stage / max(n_stage, 1)
+    stage_avg_ms = f64["stage_avg_ms"]
+    n_stage = f64["stage_launches"]
     stage_bytes = (352 + 528) / 2 * cells
-    final_avg_ms = t_final / max(n_final, 1)
+    final_avg_ms = f64["final_avg_ms"]
+    n_final = f64["final_launches"]
     final_bytes = 528 * cells
     achieved = stage_bytes / (stage_avg_ms * 1e-3) / 1e9
     traffic = None
@@ -150,14 +167,20 @@ def main():
             "stage_1_2_avg": stage_avg_ms,
             "stage_3_avg": final_avg_ms,
             "stage_3_GBs": final_bytes / (final_avg_ms * 1e-3) / 1e9 if n_final else None,
-            "ghost_fill_avg": t_ghost / max(n_ghost, 1),
+            "ghost_fill_avg": f64["ghost_avg_ms"],
         },
-        "checks": {"nonfinite": nonfinite, "ham_l2": (sumsq / count) ** 0.5 if count else None},
+        "checks": {"nonfinite": f64["nonfinite"], "ham_l2": f64["ham_l2"], "t": f64["t"]},
+        "forms": {
+            "fp64": f64,
+            "mixed": dict(mix, speedup_vs_fp64=mix["value"] / value,
+                          ham_l2_rel_vs_fp64=abs(mix["ham_l2"] - f64["ham_l2"]) / f64["ham_l2"],
+                          note="FP32 stencils/algebra on tiles with min chi >= 0.2, FP64 elsewhere; FP64 state, "
+                               "combine, metric inverse and monitor; same HBM bytes as FP64 (1408 B/cell-step)"),
+        },
     }
     if cpu.get("value"):
         line["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(line), flush=True)
-    lvl.close()
     eng.close()
 
 

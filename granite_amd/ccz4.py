@@ -114,6 +114,8 @@ def library():
         "gr_level_constraints_l2": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
         "gr_level_nonfinite": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
         "gr_level_profile": (C.c_int, [vp, C.c_int]),
+        "gr_level_set_mixed_policy": (C.c_int, [vp, C.c_double]),
+        "gr_level_rhs": (C.c_int, [vp, C.c_int, C.POINTER(_Params), C.c_int, _dp]),
         "gr_level_profile_read": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
@@ -382,6 +384,15 @@ class Level:
         n = C.c_longlong()
         self._e.check(self._e.lib.gr_level_nonfinite(self._h, C.byref(n)))
         return n.value
+
+    def set_mixed_policy(self, chi_fp64_below):
+        self._e.check(self._e.lib.gr_level_set_mixed_policy(self._h, float(chi_fp64_below)))
+
+    def rhs(self, i, params: CCZ4Params, precision=PREC_FP64):
+        """RHS of block i's device state (host array shaped like the block)."""
+        out = np.zeros_like(self.blocks[i].array)
+        self._e.check(self._e.lib.gr_level_rhs(self._h, int(i), C.byref(params.c()), int(precision), _ptr(out)))
+        return out
 
     def profile(self, enable=True):
         self._e.check(self._e.lib.gr_level_profile(self._h, 1 if enable else 0))

@@ -38,6 +38,8 @@ struct gr_level {
     double* partial = nullptr; // reduction partials
     double* red = nullptr;     // reduction result (1 double)
     double* bh = nullptr;      // puncture masses/positions for initial data
+    std::vector<unsigned char*> flags; // mixed precision: per-tile FP64 flags of each block
+    double chi_fp64_below = 0.2;
     int partial_cap = 1024;
     // captured RK3 step
     hipGraph_t graph = nullptr;
@@ -126,7 +128,7 @@ const char* gr_version(void) { return kVersion; }
 
 int gr_stage_tile(int precision, int* t3) {
     if (!t3) return GR_ERR_ARG;
-    if (precision != GR_PREC_FP64) return GR_ERR_ARG;
+    if (precision != GR_PREC_FP64 && precision != GR_PREC_MIXED) return GR_ERR_ARG;
     gr::stage_tile_fp64(t3);
     return GR_OK;
 }
@@ -288,9 +290,12 @@ int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** 
         hipError_t st = hipMalloc((void**)&u, bytes);
         if (st == hipSuccess) st = hipMalloc((void**)&a, bytes);
         if (st == hipSuccess) st = hipMalloc((void**)&c, bytes);
+        unsigned char* fl = nullptr;
+        if (st == hipSuccess) st = hipMalloc((void**)&fl, gr::stage_tiles(b));
         L->u.push_back(u);
         L->s1.push_back(a);
         L->s2.push_back(c);
+        L->flags.push_back(fl);
         if (st != hipSuccess) {
             gr_level_destroy(L);
             return hipfail(e, st, "hipMalloc(level block)");
@@ -323,6 +328,8 @@ int gr_level_destroy(gr_level* L) {
     for (double* p : L->s1)
         if (p) (void)hipFree(p);
     for (double* p : L->s2)
+        if (p) (void)hipFree(p);
+    for (unsigned char* p : L->flags)
         if (p) (void)hipFree(p);
     if (L->ham) (void)hipFree(L->ham);
     if (L->partial) (void)hipFree(L->partial);
@@ -429,10 +436,19 @@ static hipError_t timed(gr_level* L, int kind, F&& launch) {
     return hipSuccess;
 }
 
-static hipError_t enqueue_rk3(gr_level* L, const gr_ccz4_params* prm, double dt, int bc, int pre) {
-    gr_engine* e = L->eng;
-    const gr::CellParams P = cell_params(prm);
+// one stage kernel launch in the requested precision
+static hipError_t stage(gr_level* L, size_t b, const double* in, const double* un, double* out,
+                        const gr_ccz4_params* prm, int mode, const gr::StageCoef& c, int precision) {
     const gr::Sources none{nullptr, nullptr, nullptr, nullptr};
+    const gr::CellParams P = cell_params(prm);
+    if (precision == GR_PREC_MIXED)
+        return gr::launch_stage_mixed(L->blk[b], in, un, out, none, P, prm->ko_sigma, mode, c, L->eng->stream,
+                                      L->flags[b], L->chi_fp64_below);
+    return gr::launch_stage_fp64(L->blk[b], in, un, out, none, P, prm->ko_sigma, mode, c, L->eng->stream);
+}
+
+static hipError_t enqueue_rk3(gr_level* L, const gr_ccz4_params* prm, double dt, int bc, int pre, int precision) {
+    gr_engine* e = L->eng;
     // stage weights of TimeIntegrator::sspRK3Step, src/main.cpp:673,679,685
     const gr::StageCoef c1{1.0, 0.0, 1.0 * dt};
     const gr::StageCoef c2{0.75, 0.25, 0.25 * dt};
@@ -444,19 +460,16 @@ static hipError_t enqueue_rk3(gr_level* L, const gr_ccz4_params* prm, double dt,
         hipError_t st;
         if (pre >= 0 && (st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, u, pre, s); })) != hipSuccess)
             return st;
-        if ((st = timed(L, 0, [&] {
-                 return gr::launch_stage_fp64(B, u, u, s1, none, P, prm->ko_sigma, gr::MODE_STAGE, c1, s);
-             })) != hipSuccess)
+        if ((st = timed(L, 0, [&] { return stage(L, b, u, u, s1, prm, gr::MODE_STAGE, c1, precision); })) !=
+            hipSuccess)
             return st;
         if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, s1, bc, s); })) != hipSuccess) return st;
-        if ((st = timed(L, 0, [&] {
-                 return gr::launch_stage_fp64(B, s1, u, s2, none, P, prm->ko_sigma, gr::MODE_STAGE, c2, s);
-             })) != hipSuccess)
+        if ((st = timed(L, 0, [&] { return stage(L, b, s1, u, s2, prm, gr::MODE_STAGE, c2, precision); })) !=
+            hipSuccess)
             return st;
         if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, s2, bc, s); })) != hipSuccess) return st;
-        if ((st = timed(L, 1, [&] {
-                 return gr::launch_stage_fp64(B, s2, u, u, none, P, prm->ko_sigma, gr::MODE_FINAL, c3, s);
-             })) != hipSuccess)
+        if ((st = timed(L, 1, [&] { return stage(L, b, s2, u, u, prm, gr::MODE_FINAL, c3, precision); })) !=
+            hipSuccess)
             return st;
         if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, u, bc, s); })) != hipSuccess) return st;
     }
@@ -473,11 +486,11 @@ int gr_level_rk3_step(gr_level* L, const gr_ccz4_params* prm, double dt, int pre
     if (rc) return rc;
     if ((rc = check_params(e, prm))) return rc;
     if (!std::isfinite(dt)) return fail(e, GR_ERR_ARG, "dt must be finite");
-    if (precision != GR_PREC_FP64) return fail(e, GR_ERR_ARG, "this build implements the FP64 stage kernels only");
+    if (precision != GR_PREC_FP64 && precision != GR_PREC_MIXED) return fail(e, GR_ERR_ARG, "unknown precision");
     if (bc_mode < 0 || bc_mode > 2) return fail(e, GR_ERR_ARG, "bc_mode must be 0, 1 or 2");
     if (pre_fill < -1 || pre_fill > 2) return fail(e, GR_ERR_ARG, "pre_fill must be -1, 0, 1 or 2");
     if (!use_graph || L->profile) {
-        GR_HIP(e, enqueue_rk3(L, prm, dt, bc_mode, pre_fill));
+        GR_HIP(e, enqueue_rk3(L, prm, dt, bc_mode, pre_fill, precision));
         return GR_OK;
     }
     const double key[10] = {dt, prm->kappa1, prm->kappa2, prm->eta, prm->ko_sigma, prm->chi_blend_center,
@@ -489,7 +502,7 @@ int gr_level_rk3_step(gr_level* L, const gr_ccz4_params* prm, double dt, int pre
         L->graph = nullptr;
         L->have_graph = false;
         GR_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t st = enqueue_rk3(L, prm, dt, bc_mode, pre_fill);
+        hipError_t st = enqueue_rk3(L, prm, dt, bc_mode, pre_fill, precision);
         hipGraph_t g = nullptr;
         hipError_t st2 = hipStreamEndCapture(e->stream, &g);
         if (st != hipSuccess) {
@@ -503,6 +516,30 @@ int gr_level_rk3_step(gr_level* L, const gr_ccz4_params* prm, double dt, int pre
         L->have_graph = true;
     }
     GR_HIP(e, hipGraphLaunch(L->exec, e->stream));
+    return GR_OK;
+}
+
+int gr_level_set_mixed_policy(gr_level* L, double chi_fp64_below) {
+    if (!L) return GR_ERR_ARG;
+    if (!std::isfinite(chi_fp64_below)) return fail(L->eng, GR_ERR_ARG, "threshold must be finite");
+    L->chi_fp64_below = chi_fp64_below;
+    L->have_graph = false; // the captured step bakes the threshold in
+    return GR_OK;
+}
+
+int gr_level_rhs(gr_level* L, int b, const gr_ccz4_params* prm, int precision, double* host_rhs) {
+    int rc = level_block(L, b);
+    if (rc) return rc;
+    gr_engine* e = L->eng;
+    if ((rc = check_params(e, prm))) return rc;
+    if (!host_rhs) return fail(e, GR_ERR_ARG, "null output");
+    if (precision != GR_PREC_FP64 && precision != GR_PREC_MIXED) return fail(e, GR_ERR_ARG, "unknown precision");
+    const size_t n = 22 * (size_t)L->blk[b].stride;
+    GR_HIP(e, hipMemsetAsync(L->s1[b], 0, n * sizeof(double), e->stream));
+    GR_HIP(e, stage(L, (size_t)b, L->u[b], L->u[b], L->s1[b], prm, gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0},
+                    precision));
+    GR_HIP(e, hipMemcpyAsync(host_rhs, L->s1[b], n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
     return GR_OK;
 }
 

@@ -185,10 +185,12 @@ struct CellSources {
     double rho, S, Si[3], Sij[6];
 };
 
-template <bool HAS_SRC, bool KO_EXACT, class Stager, class Sink>
-__host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<double>& K, const CellParams& P,
+// R is the arithmetic of the stencils and the RHS algebra (double for the parity form,
+// float for the mixed form). The determinant and inverse of the conformal metric are
+// always formed in double from the stager's double metric (Stager::metric64).
+template <typename R, bool HAS_SRC, bool KO_EXACT, class Stager, class Sink>
+__host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R>& K, const CellParams& P,
                                              const CellSources& src) {
-    using R = double;
     // ---- centre values and the inverse conformal metric (ccz4.cpp:279-330)
     const R chi_raw = S.centre(CHI);
     const R chi = (finite_(chi_raw) && chi_raw >= R(1.0e-4)) ? chi_raw : R(1.0e-4);
@@ -206,17 +208,21 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<d
         Gh[a] = S.centre(GHX + a);
         be[a] = S.centre(BX + a);
     }
-    const R det = gt[0] * (gt[3] * gt[5] - gt[4] * gt[4]) - gt[1] * (gt[1] * gt[5] - gt[4] * gt[2]) +
-        gt[2] * (gt[1] * gt[4] - gt[3] * gt[2]);
-    const R idet = R(1) / (det + R(1.0e-30));
     R gu[6];
-    gu[0] = (gt[3] * gt[5] - gt[4] * gt[4]) * idet;
-    gu[1] = (gt[2] * gt[4] - gt[1] * gt[5]) * idet;
-    gu[2] = (gt[1] * gt[4] - gt[2] * gt[3]) * idet;
-    gu[3] = (gt[0] * gt[5] - gt[2] * gt[2]) * idet;
-    gu[4] = (gt[1] * gt[2] - gt[0] * gt[4]) * idet;
-    gu[5] = (gt[0] * gt[3] - gt[1] * gt[1]) * idet;
-    const R w = R(0.5) * (R(1) + tanh_((chi - P.chi_c) / P.chi_w));
+    {
+        double g[6];
+        S.metric64(g);
+        const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+            g[2] * (g[1] * g[4] - g[3] * g[2]);
+        const double idet = 1.0 / (det + 1.0e-30);
+        gu[0] = R((g[3] * g[5] - g[4] * g[4]) * idet);
+        gu[1] = R((g[2] * g[4] - g[1] * g[5]) * idet);
+        gu[2] = R((g[1] * g[4] - g[2] * g[3]) * idet);
+        gu[3] = R((g[0] * g[5] - g[2] * g[2]) * idet);
+        gu[4] = R((g[1] * g[2] - g[0] * g[4]) * idet);
+        gu[5] = R((g[0] * g[3] - g[1] * g[1]) * idet);
+    }
+    const R w = R(0.5) * (R(1) + tanh_((chi - R(P.chi_c)) / R(P.chi_w)));
 
     R d1[3], dd[6], adv, ko;
 
@@ -243,7 +249,7 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<d
         pin(GA[c]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) pin(ddiv[a]);
-        O.put(BX + c, R(0.75) * Gh[c] - P.eta * be[c] + adv, ko, be[c]);
+        O.put(BX + c, R(0.75) * Gh[c] - R(P.eta) * be[c] + adv, ko, be[c]);
     }
     const R divb = dbe[0][0] + dbe[1][1] + dbe[2][2];
 #pragma unroll
@@ -317,7 +323,7 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<d
 #pragma unroll
             for (int q = 0; q < 6; ++q) T1 += ((q == 0 || q == 3 || q == 5) ? R(1) : R(2)) * chr[i][q] * At[q];
             GA[i] += R(2) * alpha * T1 + c4 * (Au[i][0] * dchi[0] + Au[i][1] * dchi[1] + Au[i][2] * dchi[2]) -
-                R(2) * P.kappa1 * alpha * Gh[i];
+                R(2) * R(P.kappa1) * alpha * Gh[i];
             pin(GA[i]);
         }
     }
@@ -405,8 +411,9 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<d
             cga += gu[q] * (l == m ? dchi[l] * dal[l] : dchi[l] * dal[m] + dchi[m] * dal[l]);
         }
         const R plap = chi * lap - R(0.5) * cga;
-        const R matter = HAS_SRC ? R(4) * R(PI) * alpha * (src.rho + src.S) : R(0);
-        O.put(KTR, -plap + alpha * (AA + Ktr * Ktr / R(3)) + matter + P.kappa1 * (R(1) - P.kappa2) * alpha * Theta + adv,
+        const R matter = HAS_SRC ? R(4) * R(PI) * alpha * (R(src.rho) + R(src.S)) : R(0);
+        O.put(KTR,
+              -plap + alpha * (AA + Ktr * Ktr / R(3)) + matter + R(P.kappa1) * (R(1) - R(P.kappa2)) * alpha * Theta + adv,
               ko, Ktr);
     }
     {
@@ -416,7 +423,7 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<d
             GA[i] += c2 * (gu[sym(i, 0)] * dK[0] + gu[sym(i, 1)] * dK[1] + gu[sym(i, 2)] * dK[2]);
             if (HAS_SRC)
                 GA[i] += -R(16) * R(PI) * alpha *
-                    (gu[sym(i, 0)] * src.Si[0] + gu[sym(i, 1)] * src.Si[1] + gu[sym(i, 2)] * src.Si[2]);
+                    (gu[sym(i, 0)] * R(src.Si[0]) + gu[sym(i, 1)] * R(src.Si[1]) + gu[sym(i, 2)] * R(src.Si[2]));
             pin(GA[i]);
         }
     }
@@ -424,10 +431,10 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<d
     // ---- Theta (ccz4.cpp:654-665)
     eval_field<false, KO_EXACT>(S.field(THETA), K, S.strides(), be, w, d1, dd, adv, ko);
     {
-        const R rho = HAS_SRC ? src.rho : R(0);
+        const R rho = HAS_SRC ? R(src.rho) : R(0);
         O.put(THETA,
               R(0.5) * alpha * (Rs - AA + R(2) * Ktr * Ktr / R(3) - R(16) * R(PI) * rho) -
-                  alpha * P.kappa1 * (R(2) + P.kappa2) * Theta + adv,
+                  alpha * R(P.kappa1) * (R(2) + R(P.kappa2)) * Theta + adv,
               ko, Theta);
     }
 
@@ -444,7 +451,7 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<d
     for (int p = 0; p < 6; ++p) {
         const int a = sym_a(p), b = sym_b(p);
         eval_field<false, KO_EXACT>(S.field(AXX + p), K, S.strides(), be, w, d1, dd, adv, koA[p]);
-        const R Sij = HAS_SRC ? src.Sij[p] : R(0);
+        const R Sij = HAS_SRC ? R(src.Sij[p]) : R(0);
         const R t1 = chi * (-DDa[p] + alpha * (Rt[p] - R(8) * R(PI) * Sij));
         const R Asq = At[sym(a, 0)] * Au[0][b] + At[sym(a, 1)] * Au[1][b] + At[sym(a, 2)] * Au[2][b];
         const R t2 = alpha * (Ktr * At[p] - R(2) * Asq);

@@ -30,6 +30,12 @@
 #ifndef GR_FP64_MIN_WAVES
 #define GR_FP64_MIN_WAVES 1
 #endif
+#ifndef GR_MIXED_MIN_WAVES
+#define GR_MIXED_MIN_WAVES 1
+#endif
+#ifndef GR_MIXED_PREFETCH
+#define GR_MIXED_PREFETCH 1
+#endif
 
 namespace gr {
 
@@ -194,6 +200,10 @@ struct TileIO {
 
     // ---- Stager
     __device__ double centre(int v) const { return in[v * stride + cidx]; }
+    __device__ void metric64(double g[6]) const {
+#pragma unroll
+        for (int a = 0; a < 6; ++a) g[a] = in[(GXX + a) * stride + cidx];
+    }
     __device__ FixedStrides<1, T::BX, T::BX * T::BY> strides() const { return {}; }
     __device__ const double* field(int v) {
         const int pos = field_pos(v);
@@ -236,6 +246,135 @@ struct TileIO {
     }
 };
 
+// Mixed-precision counterpart of TileIO: the stencils and the RHS algebra see FP32
+// values, the RK3 combine, the floors and the algebraic constraints are done in FP64 on
+// the FP64 state. Each field is staged as FP32 deltas from one FP64 reference value per
+// tile (the field at the tile's first cell): every stencil's weights sum to zero, so
+// derivatives of the deltas are derivatives of the field, while small structure on top of
+// an O(1) field (1e-8 noise on the metric) keeps FP32 relative precision of the delta.
+template <int TX, int TY, int TZ, int MODE, int PF>
+struct TileIO32 {
+    using T = TileGeom<TX, TY, TZ>;
+    static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
+    static constexpr int LBUF = T::NLDS + 1;
+
+    float* lds; // 2 * LBUF floats
+    const double* in;
+    const double* un;
+    double* out;
+    long stride;
+    long cidx;
+    long ridx; // the tile's reference cell
+    int cl;
+    bool active;
+    bool ko_on;
+    StageCoef cf;
+    int loff[NL];
+    int goff[NL];
+    double buf[PF][NL];
+    double refb[PF], unb[PF], inb[PF];
+    double unv[NV], inv[NV];
+    double hg[6], hA[6];
+
+    __device__ void init(int tid, int ox, int oy, int oz, int nt0, int nt1, int nt2) {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const int p = tid + i * T::NT;
+            int b = 0, q = p;
+#pragma unroll
+            for (int k = 1; k < T::NBOX; ++k)
+                if (p >= T::start(k)) {
+                    b = k;
+                    q = p - T::start(k);
+                }
+            const int nx = T::box[b][1] - T::box[b][0];
+            const int ny = T::box[b][3] - T::box[b][2];
+            const int x = T::box[b][0] + q % nx;
+            const int y = T::box[b][2] + (q / nx) % ny;
+            const int z = T::box[b][4] + q / (nx * ny);
+            const int gx = ox + x, gy = oy + y, gz = oz + z;
+            const bool ok = p < T::TOTAL && gx < nt0 && gy < nt1 && gz < nt2;
+            loff[i] = ok ? T::lds(x, y, z) : T::NLDS;
+            goff[i] = ok ? gx + nt0 * (gy + nt1 * gz) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) prefetch(k);
+        stage_to_lds(0);
+        prefetch(PF);
+    }
+
+    __device__ __forceinline__ void prefetch(int pos) {
+        if (pos >= NV) return;
+        const int v = FIELD_ORDER[pos];
+        const int slot = pos % PF;
+        const double* f = in + v * stride;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) buf[slot][i] = f[goff[i]];
+        refb[slot] = f[ridx];
+        inb[slot] = f[cidx];
+        if (MODE != MODE_RHS) unb[slot] = un[v * stride + cidx];
+    }
+
+    __device__ __forceinline__ void stage_to_lds(int pos) {
+        if (pos >= NV) return;
+        const int slot = pos % PF;
+        float* L = lds + (pos & 1) * LBUF;
+        const double r = refb[slot];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) L[loff[i]] = (float)(buf[slot][i] - r);
+        const int v = FIELD_ORDER[pos];
+        inv[v] = inb[slot];
+        if (MODE != MODE_RHS) unv[v] = unb[slot];
+    }
+
+    // ---- Stager
+    __device__ float centre(int v) const { return (float)in[v * stride + cidx]; }
+    __device__ void metric64(double g[6]) const {
+#pragma unroll
+        for (int a = 0; a < 6; ++a) g[a] = in[(GXX + a) * stride + cidx];
+    }
+    __device__ FixedStrides<1, T::BX, T::BX * T::BY> strides() const { return {}; }
+    __device__ const float* field(int v) {
+        const int pos = field_pos(v);
+        __syncthreads();
+        stage_to_lds(pos + 1);
+        prefetch(pos + 1 + PF);
+        return lds + (pos & 1) * LBUF + cl;
+    }
+
+    // ---- Sink (FP64 combine on the FP64 stage input and u^n)
+    __device__ __forceinline__ void put(int v, float r, float ko, float) {
+        const double rr = ko_on ? (double)r + (double)ko : (double)r;
+        if (MODE == MODE_RHS) {
+            if (active) out[v * stride + cidx] = rr;
+            return;
+        }
+        double val = combine_exact(cf.a1, unv[v], cf.a2, inv[v], cf.a3dt, rr);
+        if (v == CHI) val = floor_chi(val);
+        if (v == ALPHA) val = floor_alpha(val);
+        if (MODE == MODE_FINAL && v >= GXX && v < GXX + 6) {
+            hg[v - GXX] = val;
+            return;
+        }
+        if (MODE == MODE_FINAL && v >= AXX && v < AXX + 6) {
+            hA[v - AXX] = val;
+            return;
+        }
+        if (active) out[v * stride + cidx] = val;
+    }
+
+    __device__ __forceinline__ void finish() {
+        if (MODE != MODE_FINAL) return;
+        algebraic_exact(hg, hA);
+        if (!active) return;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            out[(GXX + a) * stride + cidx] = hg[a];
+            out[(AXX + a) * stride + cidx] = hA[a];
+        }
+    }
+};
+
 struct StageArgs {
     const double* in;
     const double* un;
@@ -249,9 +388,13 @@ struct StageArgs {
     int ntiles;
     int xcd_remap;
     Stencil<double> K;
+    Stencil<float> K32;
     CellParams P;
     StageCoef cf;
     int ko_on;
+    // tiles whose flag differs from `want` are left to the other precision's kernel
+    const unsigned char* flags;
+    int want;
 };
 
 // workgroup id -> tile id: consecutive tiles go to the same XCD (workgroups are dealt
@@ -269,6 +412,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(St
     __shared__ double lds[2 * (T::NLDS + 1)];
 
     const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
+    if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
     const int tix = tile % A.tiles[0];
     const int tiy = (tile / A.tiles[0]) % A.tiles[1];
     const int tiz = tile / (A.tiles[0] * A.tiles[1]);
@@ -301,10 +445,73 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(St
 #pragma unroll
         for (int a = 0; a < 6; ++a) src.Sij[a] = A.src.Sij[6 * cidx + a];
     }
-    ccz4_cell<HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
+    ccz4_cell<double, HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
 }
 
 constexpr int FP64_TX = GR_FP64_TILE_X, FP64_TY = GR_FP64_TILE_Y, FP64_TZ = GR_FP64_TILE_Z;
+
+template <int TX, int TY, int TZ, int MODE, bool HAS_SRC>
+__global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_mixed(StageArgs A) {
+    using T = TileGeom<TX, TY, TZ>;
+    __shared__ float lds[2 * (T::NLDS + 1)];
+
+    const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
+    if (A.flags && A.flags[tile] != A.want) return;
+    const int tix = tile % A.tiles[0];
+    const int tiy = (tile / A.tiles[0]) % A.tiles[1];
+    const int tiz = tile / (A.tiles[0] * A.tiles[1]);
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
+    const int ix = tix * TX + lx, iy = tiy * TY + ly, iz = tiz * TZ + lz;
+    const bool active = ix < A.n[0] && iy < A.n[1] && iz < A.n[2];
+    const int gi = A.ng + (active ? ix : 0), gj = A.ng + (active ? iy : 0), gk = A.ng + (active ? iz : 0);
+    const long cidx = gi + (long)A.nt[0] * (gj + (long)A.nt[1] * gk);
+
+    TileIO32<TX, TY, TZ, MODE, GR_MIXED_PREFETCH> io;
+    io.lds = lds;
+    io.in = A.in;
+    io.un = A.un;
+    io.out = A.out;
+    io.stride = A.stride;
+    io.cidx = cidx;
+    io.ridx = (A.ng + tix * TX) + (long)A.nt[0] * ((A.ng + tiy * TY) + (long)A.nt[1] * (A.ng + tiz * TZ));
+    io.cl = T::lds(lx, ly, lz);
+    io.active = active;
+    io.ko_on = A.ko_on != 0;
+    io.cf = A.cf;
+    io.init(tid, A.ng + tix * TX, A.ng + tiy * TY, A.ng + tiz * TZ, A.nt[0], A.nt[1], A.nt[2]);
+
+    CellSources src{};
+    if (HAS_SRC) {
+        src.rho = A.src.rho[cidx];
+        src.S = A.src.S[cidx];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) src.Si[a] = A.src.Si[3 * cidx + a];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) src.Sij[a] = A.src.Sij[6 * cidx + a];
+    }
+    ccz4_cell<float, HAS_SRC, false>(io, io, A.K32, A.P, src);
+}
+
+// flags[tile] = 1 where the tile needs the FP64 kernel in the mixed form: its smallest chi
+// (interior cells of the stage input) is below chi_min or not finite.
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(TX* TY* TZ) void classify_tiles_kernel(BlockDesc b, const double* u, int t0, int t1,
+                                                                      double chi_min, unsigned char* flags) {
+    __shared__ int bad;
+    const int tile = blockIdx.x;
+    const int tix = tile % t0, tiy = (tile / t0) % t1, tiz = tile / (t0 * t1);
+    const int tid = threadIdx.x;
+    const int ix = tix * TX + tid % TX, iy = tiy * TY + (tid / TX) % TY, iz = tiz * TZ + tid / (TX * TY);
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    if (ix < b.n[0] && iy < b.n[1] && iz < b.n[2]) {
+        const double c = u[CHI * b.stride + (b.ng + ix) + (long)b.nt[0] * ((b.ng + iy) + (long)b.nt[1] * (b.ng + iz))];
+        if (!(c >= chi_min)) bad = 1; // benign race: every writer stores 1
+    }
+    __syncthreads();
+    if (tid == 0) flags[tile] = (unsigned char)bad;
+}
 
 template <int MODE, bool HAS_SRC>
 hipError_t launch_stage_t(const StageArgs& A, int nblk, hipStream_t s) {
@@ -719,8 +926,13 @@ void stage_tile_fp64(int* t3) {
     t3[2] = FP64_TZ;
 }
 
-hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
-                             const CellParams& P, double ko_sigma, int mode, const StageCoef& cf, hipStream_t s) {
+#ifndef GR_XCD_REMAP
+#define GR_XCD_REMAP 1
+#endif
+
+static StageArgs stage_args(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
+                            const CellParams& P, double ko_sigma, const StageCoef& cf, const unsigned char* flags,
+                            int want) {
     StageArgs A;
     A.in = in;
     A.un = un;
@@ -736,14 +948,32 @@ hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double*
     A.tiles[1] = (b.n[1] + FP64_TY - 1) / FP64_TY;
     A.tiles[2] = (b.n[2] + FP64_TZ - 1) / FP64_TZ;
     A.ntiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
-#ifndef GR_XCD_REMAP
-#define GR_XCD_REMAP 1
-#endif
     A.xcd_remap = (GR_XCD_REMAP && A.ntiles % 8 == 0) ? 1 : 0;
     A.K = make_stencil(b, ko_sigma);
+    for (int d = 0; d < 3; ++d) {
+        A.K32.ih[d] = (float)A.K.ih[d];
+        A.K32.ih2[d] = (float)A.K.ih2[d];
+        A.K32.ihh[d] = (float)A.K.ihh[d];
+        A.K32.ko[d] = (float)A.K.ko[d];
+        for (int t = 0; t < 7; ++t) A.K32.kw[d][t] = (float)A.K.kw[d][t];
+    }
     A.P = P;
     A.cf = cf;
     A.ko_on = ko_sigma > 0.0 ? 1 : 0;
+    A.flags = flags;
+    A.want = want;
+    return A;
+}
+
+int stage_tiles(const BlockDesc& b) {
+    return ((b.n[0] + FP64_TX - 1) / FP64_TX) * ((b.n[1] + FP64_TY - 1) / FP64_TY) *
+        ((b.n[2] + FP64_TZ - 1) / FP64_TZ);
+}
+
+hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
+                             const CellParams& P, double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
+                             const unsigned char* flags) {
+    const StageArgs A = stage_args(b, in, un, out, src, P, ko_sigma, cf, flags, 1);
     const bool has_src = src.rho != nullptr;
     if (has_src) {
         if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, true>(A, A.ntiles, s);
@@ -753,6 +983,39 @@ hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double*
     if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, false>(A, A.ntiles, s);
     if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, false>(A, A.ntiles, s);
     return launch_stage_t<MODE_FINAL, false>(A, A.ntiles, s);
+}
+
+template <int MODE, bool HAS_SRC>
+static hipError_t launch_mixed_t(const StageArgs& A, hipStream_t s) {
+    stage_kernel_mixed<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC>
+        <<<dim3(A.ntiles), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_mixed(const BlockDesc& b, const double* in, const double* un, double* out,
+                              const Sources& src, const CellParams& P, double ko_sigma, int mode,
+                              const StageCoef& cf, hipStream_t s, unsigned char* flags, double chi_fp64_below) {
+    const int nt = stage_tiles(b);
+    const int t0 = (b.n[0] + FP64_TX - 1) / FP64_TX, t1 = (b.n[1] + FP64_TY - 1) / FP64_TY;
+    classify_tiles_kernel<FP64_TX, FP64_TY, FP64_TZ>
+        <<<dim3(nt), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(b, in, t0, t1, chi_fp64_below, flags);
+    hipError_t st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    // FP32 tiles (flag 0)
+    const StageArgs A = stage_args(b, in, un, out, src, P, ko_sigma, cf, flags, 0);
+    const bool has_src = src.rho != nullptr;
+    if (has_src) {
+        if (mode == MODE_RHS) st = launch_mixed_t<MODE_RHS, true>(A, s);
+        else if (mode == MODE_STAGE) st = launch_mixed_t<MODE_STAGE, true>(A, s);
+        else st = launch_mixed_t<MODE_FINAL, true>(A, s);
+    } else {
+        if (mode == MODE_RHS) st = launch_mixed_t<MODE_RHS, false>(A, s);
+        else if (mode == MODE_STAGE) st = launch_mixed_t<MODE_STAGE, false>(A, s);
+        else st = launch_mixed_t<MODE_FINAL, false>(A, s);
+    }
+    if (st != hipSuccess) return st;
+    // FP64 tiles near the puncture (flag 1)
+    return launch_stage_fp64(b, in, un, out, src, P, ko_sigma, mode, cf, s, flags);
 }
 
 hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s) {

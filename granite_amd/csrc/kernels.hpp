@@ -39,7 +39,17 @@ Stencil<double> make_stencil(const BlockDesc& b, double ko_sigma);
 // RHS / RK3 stage kernel on one block (FP64). in: stencil input (ghosts filled);
 // un: u^n for the combine (ignored for MODE_RHS); out: RHS (MODE_RHS) or the new state.
 hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
-                             const CellParams& P, double ko_sigma, int mode, const StageCoef& cf, hipStream_t s);
+                             const CellParams& P, double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
+                             const unsigned char* flags = nullptr);
+
+// Mixed-precision stage: classifies the tiles of `in` (FP64 where the smallest chi of the
+// tile is below chi_fp64_below), then runs the FP32 kernel on the other tiles and the FP64
+// kernel on those. flags: device buffer of stage_tiles(b) bytes.
+hipError_t launch_stage_mixed(const BlockDesc& b, const double* in, const double* un, double* out,
+                              const Sources& src, const CellParams& P, double ko_sigma, int mode,
+                              const StageCoef& cf, hipStream_t s, unsigned char* flags, double chi_fp64_below);
+// number of stage tiles of a block
+int stage_tiles(const BlockDesc& b);
 
 // Ghost fill of one block: mode 0 Sommerfeld (chained X,Y,Z), 1 periodic, 2 outflow copy.
 hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s);

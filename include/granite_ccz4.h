@@ -144,6 +144,16 @@ int gr_level_fill_ghosts(gr_level* lvl, int mode);
 int gr_level_rk3_step(gr_level* lvl, const gr_ccz4_params* prm, double dt, int precision, int bc_mode,
                       int pre_fill, int use_graph);
 
+/* Mixed-precision policy: tiles whose smallest chi is below chi_fp64_below (or not
+ * finite) run the FP64 stage kernel; all other tiles the FP32 one. Default 0.2. */
+int gr_level_set_mixed_policy(gr_level* lvl, double chi_fp64_below);
+
+/* RHS (with dissipation) of block `block`'s current state in the given precision, written
+ * to host_rhs (22*stride doubles, interior cells; ghost cells set to 0). The FP64 form is
+ * CCZ4Evolution::computeRHSVacuum (src/spacetime/ccz4.cpp:229-245) on device-resident data;
+ * the MIXED form is the arithmetic the mixed-precision RK3 stages use. */
+int gr_level_rhs(gr_level* lvl, int block, const gr_ccz4_params* prm, int precision, double* host_rhs);
+
 /* Wait for the level's queued work. */
 int gr_level_sync(gr_level* lvl);
 

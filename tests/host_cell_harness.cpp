@@ -16,6 +16,9 @@ struct HostStager {
     gr::RuntimeStrides st;
     const double* field(int v) const { return u + v * stride + c; }
     const gr::RuntimeStrides& strides() const { return st; }
+    void metric64(double g[6]) const {
+        for (int a = 0; a < 6; ++a) g[a] = u[(gr::GXX + a) * stride + c];
+    }
 };
 
 struct HostSink {
@@ -57,9 +60,9 @@ static void run(const int* n, int ng, const double* dx, const double* prm, const
                     src.S = S[c];
                     for (int a = 0; a < 3; ++a) src.Si[a] = Si[3 * c + a];
                     for (int a = 0; a < 6; ++a) src.Sij[a] = Sij[6 * c + a];
-                    gr::ccz4_cell<true, KO_EXACT>(st, sk, K, P, src);
+                    gr::ccz4_cell<double, true, KO_EXACT>(st, sk, K, P, src);
                 } else {
-                    gr::ccz4_cell<false, KO_EXACT>(st, sk, K, P, src);
+                    gr::ccz4_cell<double, false, KO_EXACT>(st, sk, K, P, src);
                 }
             }
 }
