@@ -46,6 +46,79 @@ def cpu_baseline():
     return res
 
 
+def copy_calibration(nbytes):
+    """Five plain device-to-device copies of nbytes (the HIP runtime's copy kernel): the
+    FETCH_SIZE / WRITE_SIZE counters of one such copy read ~nbytes each when the counters
+    report HBM bytes exactly; their ratio to nbytes is the calibration factor."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a = ctypes.c_void_p()
+    b = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(5):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 5}), flush=True)
+
+
+def run_batched(ga, eng, p, args):
+    """Uneven AMR blocks (granite_amd.workloads.uneven_amr_levels: ~250 blocks, edges in
+    {16..64}, 4 levels): one RK3 step of every level (each with dt = 0.25 dx_l), either
+    with one stage launch per level (batched; FP64 and mixed) or with every block in its
+    own level (per-block launches, FP64). Value = interior cells / time of that step."""
+    from granite_amd import workloads
+
+    layout = workloads.uneven_amr_levels()
+    total = workloads.cells(layout)
+    masses, pos = workloads.PUNCTURES
+    steps = max(3, args.steps // 5)
+    out = {"blocks": sum(len(b) for _, b in layout), "levels": len(layout), "cells": total}
+
+    def timed(levels, precision):
+        for lv, dx in levels:
+            lv.init_brill_lindquist(masses, pos)
+        for _ in range(2):
+            for lv, dx in levels:
+                lv.rk3_step(p, 0.25 * dx, precision=precision, pre_fill=ga.BC_OUTFLOW, use_graph=True)
+        for lv, _ in levels:
+            lv.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for lv, dx in levels:
+                lv.rk3_step(p, 0.25 * dx, precision=precision, pre_fill=ga.BC_OUTFLOW, use_graph=True)
+        for lv, _ in levels:
+            lv.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        bad = sum(lv.nonfinite() for lv, _ in levels)
+        return {"ms_per_step": ms, "value": total / (ms * 1e-3), "nonfinite": bad}
+
+    def make(batched):
+        levels = []
+        for dx, blocks in layout:
+            grids = [ga.GridBlock(i, 0, n, lo, hi, 4) for i, (n, lo, hi) in enumerate(blocks)]
+            if batched:
+                levels.append((ga.Level(grids, eng), dx))
+            else:
+                levels += [(ga.Level([g], eng), dx) for g in grids]
+        return levels
+
+    for name, batched, prec in (("fp64_batched", True, ga.PREC_FP64), ("mixed_batched", True, ga.PREC_MIXED),
+                                ("fp64_per_block", False, ga.PREC_FP64)):
+        levels = make(batched)
+        out[name] = timed(levels, prec)
+        for lv, _ in levels:
+            lv.close()
+    out["batched_speedup_vs_per_block"] = out["fp64_per_block"]["ms_per_step"] / out["fp64_batched"]["ms_per_step"]
+    out["mixed_batched_vs_fp64_per_block"] = out["fp64_per_block"]["ms_per_step"] / out["mixed_batched"]["ms_per_step"]
+    out["steps"] = steps
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,7 +126,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-AMR-blocks section")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the level would shard "
               "per block across GPUs; see the sharding design). Run with --gpus 1.", file=sys.stderr)
@@ -105,6 +184,7 @@ def main():
 
     f64 = run_form(ga.PREC_FP64)
     mix = run_form(ga.PREC_MIXED)
+    batched = None if args.no_batched else run_batched(ga, eng, p, args)
     value = f64["value"]
     ms_per_step = f64["ms_per_step"]
 
@@ -170,6 +250,7 @@ def main():
             "ghost_fill_avg": f64["ghost_avg_ms"],
         },
         "checks": {"nonfinite": f64["nonfinite"], "ham_l2": f64["ham_l2"], "t": f64["t"]},
+        "batched_amr_blocks": batched,
         "forms": {
             "fp64": f64,
             "mixed": dict(mix, speedup_vs_fp64=mix["value"] / value,

@@ -38,8 +38,14 @@ struct gr_level {
     double* partial = nullptr; // reduction partials
     double* red = nullptr;     // reduction result (1 double)
     double* bh = nullptr;      // puncture masses/positions for initial data
-    std::vector<unsigned char*> flags; // mixed precision: per-tile FP64 flags of each block
+    unsigned char* flags = nullptr; // mixed precision: per-tile FP64 flags (all tiles of the level)
     double chi_fp64_below = 0.2;
+    // batched launches over all blocks: device block table, tile -> block map, ghost prefix
+    std::vector<gr::BatchBlock> table;
+    gr::BatchBlock* d_table = nullptr;
+    int* d_tile_block = nullptr;
+    long* d_ghost_ofs = nullptr;
+    gr::Batch batch;
     int partial_cap = 1024;
     // captured RK3 step
     hipGraph_t graph = nullptr;
@@ -200,8 +206,9 @@ int gr_ccz4_rhs(gr_engine* e, const gr_geom* geom, const gr_ccz4_params* prm, in
     }
     GR_HIP(e, hipMemcpyAsync(e->d_u, u, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
     GR_HIP(e, hipMemcpyAsync(e->d_r, rhs, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
-    GR_HIP(e, gr::launch_stage_fp64(b, e->d_u, e->d_u, e->d_r, src, cell_params(prm), prm->ko_sigma, gr::MODE_RHS,
-                                    gr::StageCoef{0.0, 0.0, 0.0}, e->stream));
+    static const int sel[3] = {0, 1, 2};
+    GR_HIP(e, gr::launch_stage_fp64(gr::single_batch(b, e->d_u, e->d_u, e->d_r), sel, src, cell_params(prm),
+                                    prm->ko_sigma, gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0}, e->stream));
     GR_HIP(e, hipMemcpyAsync(rhs, e->d_r, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     GR_HIP(e, hipStreamSynchronize(e->stream));
     return GR_OK;
@@ -219,8 +226,9 @@ int gr_ccz4_rhs_device(gr_engine* e, const gr_geom* geom, const gr_ccz4_params* 
     const int nsrc = (d_rho != nullptr) + (d_Si != nullptr) + (d_Sij != nullptr) + (d_S != nullptr);
     if (nsrc != 0 && nsrc != 4) return fail(e, GR_ERR_ARG, "matter sources must be all null or all non-null");
     gr::Sources src{d_rho, d_Si, d_Sij, d_S};
-    GR_HIP(e, gr::launch_stage_fp64(b, d_u, d_u, d_rhs, src, cell_params(prm), prm->ko_sigma, gr::MODE_RHS,
-                                    gr::StageCoef{0.0, 0.0, 0.0}, e->stream));
+    static const int sel[3] = {0, 1, 2};
+    GR_HIP(e, gr::launch_stage_fp64(gr::single_batch(b, d_u, d_u, d_rhs), sel, src, cell_params(prm), prm->ko_sigma,
+                                    gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0}, e->stream));
     return GR_OK;
 }
 
@@ -290,20 +298,57 @@ int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** 
         hipError_t st = hipMalloc((void**)&u, bytes);
         if (st == hipSuccess) st = hipMalloc((void**)&a, bytes);
         if (st == hipSuccess) st = hipMalloc((void**)&c, bytes);
-        unsigned char* fl = nullptr;
-        if (st == hipSuccess) st = hipMalloc((void**)&fl, gr::stage_tiles(b));
         L->u.push_back(u);
         L->s1.push_back(a);
         L->s2.push_back(c);
-        L->flags.push_back(fl);
         if (st != hipSuccess) {
             gr_level_destroy(L);
             return hipfail(e, st, "hipMalloc(level block)");
         }
     }
+    // one batched launch per stage needs one grid spacing for the level
+    for (int i = 1; i < nblocks; ++i)
+        for (int d = 0; d < 3; ++d)
+            if (L->blk[i].dx[d] != L->blk[0].dx[d]) {
+                gr_level_destroy(L);
+                return fail(e, GR_ERR_GEOM, "all blocks of a level must share dx");
+            }
+    L->table.resize(nblocks);
+    for (int i = 0; i < nblocks; ++i) {
+        L->table[i].b = L->blk[i];
+        L->table[i].buf[0] = L->u[i];
+        L->table[i].buf[1] = L->s1[i];
+        L->table[i].buf[2] = L->s2[i];
+    }
+    std::vector<int> tile_block;
+    std::vector<long> ghost_ofs;
+    const int ntiles = gr::batch_layout(L->table, tile_block, ghost_ofs);
     hipError_t st = hipMalloc((void**)&L->ham, maxstride * sizeof(double));
     if (st == hipSuccess) st = hipMalloc((void**)&L->partial, L->partial_cap * sizeof(double));
     if (st == hipSuccess) st = hipMalloc((void**)&L->red, sizeof(double));
+    if (st == hipSuccess) st = hipMalloc((void**)&L->flags, ntiles);
+    if (st == hipSuccess) st = hipMalloc((void**)&L->d_table, nblocks * sizeof(gr::BatchBlock));
+    if (st == hipSuccess) st = hipMalloc((void**)&L->d_tile_block, ntiles * sizeof(int));
+    if (st == hipSuccess) st = hipMalloc((void**)&L->d_ghost_ofs, (nblocks + 1) * sizeof(long));
+    if (st == hipSuccess)
+        st = hipMemcpy(L->d_table, L->table.data(), nblocks * sizeof(gr::BatchBlock), hipMemcpyHostToDevice);
+    if (st == hipSuccess)
+        st = hipMemcpy(L->d_tile_block, tile_block.data(), ntiles * sizeof(int), hipMemcpyHostToDevice);
+    if (st == hipSuccess)
+        st = hipMemcpy(L->d_ghost_ofs, ghost_ofs.data(), (nblocks + 1) * sizeof(long), hipMemcpyHostToDevice);
+    if (nblocks == 1) {
+        L->batch = gr::single_batch(L->blk[0], L->u[0], L->s1[0], L->s2[0]);
+    } else {
+        gr::Batch& B = L->batch;
+        B.table = L->d_table;
+        B.tile_block = L->d_tile_block;
+        B.ghost_ofs = L->d_ghost_ofs;
+        B.nblocks = nblocks;
+        B.one = L->table[0];
+        B.ntiles = ntiles;
+        B.nghost = ghost_ofs.back();
+        for (int d = 0; d < 3; ++d) B.dx[d] = L->blk[0].dx[d];
+    }
     if (st != hipSuccess) {
         gr_level_destroy(L);
         return hipfail(e, st, "hipMalloc(level scratch)");
@@ -329,8 +374,10 @@ int gr_level_destroy(gr_level* L) {
         if (p) (void)hipFree(p);
     for (double* p : L->s2)
         if (p) (void)hipFree(p);
-    for (unsigned char* p : L->flags)
-        if (p) (void)hipFree(p);
+    if (L->flags) (void)hipFree(L->flags);
+    if (L->d_table) (void)hipFree(L->d_table);
+    if (L->d_tile_block) (void)hipFree(L->d_tile_block);
+    if (L->d_ghost_ofs) (void)hipFree(L->d_ghost_ofs);
     if (L->ham) (void)hipFree(L->ham);
     if (L->partial) (void)hipFree(L->partial);
     if (L->red) (void)hipFree(L->red);
@@ -410,7 +457,7 @@ int gr_level_fill_ghosts(gr_level* L, int mode) {
     int rc = use_device(e);
     if (rc) return rc;
     if (mode < 0 || mode > 2) return fail(e, GR_ERR_ARG, "ghost mode must be 0, 1 or 2");
-    for (size_t b = 0; b < L->blk.size(); ++b) GR_HIP(e, gr::launch_fill_ghosts(L->blk[b], L->u[b], mode, e->stream));
+    GR_HIP(e, gr::launch_fill_ghosts_batch(L->batch, 0, mode, e->stream));
     return GR_OK;
 }
 
@@ -436,43 +483,39 @@ static hipError_t timed(gr_level* L, int kind, F&& launch) {
     return hipSuccess;
 }
 
-// one stage kernel launch in the requested precision
-static hipError_t stage(gr_level* L, size_t b, const double* in, const double* un, double* out,
-                        const gr_ccz4_params* prm, int mode, const gr::StageCoef& c, int precision) {
+// one stage over every block of a level (or over the single-block batch B) in the
+// requested precision; sel = {input, u^n, output} buffer indices
+static hipError_t stage(gr_level* L, const gr::Batch& B, const int sel[3], const gr_ccz4_params* prm, int mode,
+                        const gr::StageCoef& c, int precision) {
     const gr::Sources none{nullptr, nullptr, nullptr, nullptr};
     const gr::CellParams P = cell_params(prm);
     if (precision == GR_PREC_MIXED)
-        return gr::launch_stage_mixed(L->blk[b], in, un, out, none, P, prm->ko_sigma, mode, c, L->eng->stream,
-                                      L->flags[b], L->chi_fp64_below);
-    return gr::launch_stage_fp64(L->blk[b], in, un, out, none, P, prm->ko_sigma, mode, c, L->eng->stream);
+        return gr::launch_stage_mixed(B, sel, none, P, prm->ko_sigma, mode, c, L->eng->stream, L->flags,
+                                      L->chi_fp64_below);
+    return gr::launch_stage_fp64(B, sel, none, P, prm->ko_sigma, mode, c, L->eng->stream);
 }
 
 static hipError_t enqueue_rk3(gr_level* L, const gr_ccz4_params* prm, double dt, int bc, int pre, int precision) {
-    gr_engine* e = L->eng;
+    hipStream_t s = L->eng->stream;
+    const gr::Batch& B = L->batch;
     // stage weights of TimeIntegrator::sspRK3Step, src/main.cpp:673,679,685
     const gr::StageCoef c1{1.0, 0.0, 1.0 * dt};
     const gr::StageCoef c2{0.75, 0.25, 0.25 * dt};
     const gr::StageCoef c3{1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt};
-    for (size_t b = 0; b < L->blk.size(); ++b) {
-        const gr::BlockDesc& B = L->blk[b];
-        double *u = L->u[b], *s1 = L->s1[b], *s2 = L->s2[b];
-        hipStream_t s = e->stream;
-        hipError_t st;
-        if (pre >= 0 && (st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, u, pre, s); })) != hipSuccess)
-            return st;
-        if ((st = timed(L, 0, [&] { return stage(L, b, u, u, s1, prm, gr::MODE_STAGE, c1, precision); })) !=
-            hipSuccess)
-            return st;
-        if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, s1, bc, s); })) != hipSuccess) return st;
-        if ((st = timed(L, 0, [&] { return stage(L, b, s1, u, s2, prm, gr::MODE_STAGE, c2, precision); })) !=
-            hipSuccess)
-            return st;
-        if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, s2, bc, s); })) != hipSuccess) return st;
-        if ((st = timed(L, 1, [&] { return stage(L, b, s2, u, u, prm, gr::MODE_FINAL, c3, precision); })) !=
-            hipSuccess)
-            return st;
-        if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts(B, u, bc, s); })) != hipSuccess) return st;
-    }
+    // buffers: 0 = u, 1 = s1, 2 = s2
+    static const int sel1[3] = {0, 0, 1}, sel2[3] = {1, 0, 2}, sel3[3] = {2, 0, 0};
+    hipError_t st;
+    if (pre >= 0 && (st = timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, 0, pre, s); })) != hipSuccess)
+        return st;
+    if ((st = timed(L, 0, [&] { return stage(L, B, sel1, prm, gr::MODE_STAGE, c1, precision); })) != hipSuccess)
+        return st;
+    if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, 1, bc, s); })) != hipSuccess) return st;
+    if ((st = timed(L, 0, [&] { return stage(L, B, sel2, prm, gr::MODE_STAGE, c2, precision); })) != hipSuccess)
+        return st;
+    if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, 2, bc, s); })) != hipSuccess) return st;
+    if ((st = timed(L, 1, [&] { return stage(L, B, sel3, prm, gr::MODE_FINAL, c3, precision); })) != hipSuccess)
+        return st;
+    if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, 0, bc, s); })) != hipSuccess) return st;
     return hipSuccess;
 }
 
@@ -536,8 +579,9 @@ int gr_level_rhs(gr_level* L, int b, const gr_ccz4_params* prm, int precision, d
     if (precision != GR_PREC_FP64 && precision != GR_PREC_MIXED) return fail(e, GR_ERR_ARG, "unknown precision");
     const size_t n = 22 * (size_t)L->blk[b].stride;
     GR_HIP(e, hipMemsetAsync(L->s1[b], 0, n * sizeof(double), e->stream));
-    GR_HIP(e, stage(L, (size_t)b, L->u[b], L->u[b], L->s1[b], prm, gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0},
-                    precision));
+    static const int sel[3] = {0, 1, 2};
+    GR_HIP(e, stage(L, gr::single_batch(L->blk[b], L->u[b], L->u[b], L->s1[b]), sel, prm, gr::MODE_RHS,
+                    gr::StageCoef{0.0, 0.0, 0.0}, precision));
     GR_HIP(e, hipMemcpyAsync(host_rhs, L->s1[b], n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     GR_HIP(e, hipStreamSynchronize(e->stream));
     return GR_OK;

@@ -33,6 +33,9 @@
 #ifndef GR_MIXED_MIN_WAVES
 #define GR_MIXED_MIN_WAVES 1
 #endif
+#ifndef GR_LDS_PAD
+#define GR_LDS_PAD 1
+#endif
 #ifndef GR_MIXED_PREFETCH
 #define GR_MIXED_PREFETCH 1
 #endif
@@ -92,10 +95,14 @@ __device__ __forceinline__ void algebraic_exact(double g[6], double A[6]) {
 // ------------------------------------------------------------------------------------
 // LDS staging of one field of a tile
 // ------------------------------------------------------------------------------------
-template <int TX, int TY, int TZ>
+template <int TX, int TY, int TZ, bool PAD = true>
 struct TileGeom {
     static constexpr int NT = TX * TY * TZ;
-    static constexpr int BX = TX + 2 * HALO, BY = TY + 2 * HALO, BZ = TZ + 2 * HALO;
+    static constexpr int BX0 = TX + 2 * HALO, BY = TY + 2 * HALO, BZ = TZ + 2 * HALO;
+    // LDS row pitch: a wave covers 8 consecutive rows of one z-plane; a pitch of 8 mod 16
+    // elements puts those rows on disjoint bank ranges (32 or 64 four-byte banks, 4- or
+    // 8-byte elements) instead of all rows starting on the same bank
+    static constexpr int BX = (GR_LDS_PAD && PAD) ? BX0 + ((8 - BX0 % 16) + 16) % 16 : BX0;
     static constexpr int NLDS = BX * BY * BZ;
     // sub-boxes of the halo region that some stencil reads (x0,x1,y0,y1,z0,z1)
     static constexpr int NBOX = 7;
@@ -254,7 +261,8 @@ struct TileIO {
 // an O(1) field (1e-8 noise on the metric) keeps FP32 relative precision of the delta.
 template <int TX, int TY, int TZ, int MODE, int PF>
 struct TileIO32 {
-    using T = TileGeom<TX, TY, TZ>;
+    // the FP32 tile measured faster without the row padding the FP64 tile uses
+    using T = TileGeom<TX, TY, TZ, false>;
     static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
     static constexpr int LBUF = T::NLDS + 1;
 
@@ -376,15 +384,9 @@ struct TileIO32 {
 };
 
 struct StageArgs {
-    const double* in;
-    const double* un;
-    double* out;
+    Batch batch;
+    int sel_in, sel_un, sel_out;
     Sources src;
-    int n[3];
-    int nt[3];
-    long stride;
-    int ng;
-    int tiles[3];
     int ntiles;
     int xcd_remap;
     Stencil<double> K;
@@ -406,6 +408,48 @@ __device__ __forceinline__ int tile_of_block(int bid, int ntiles, int remap) {
     return (bid & 7) * per + (bid >> 3);
 }
 
+// the block and local tile coordinates of global tile t (uniform per workgroup)
+struct TileRef {
+    BlockDesc b;
+    const double* in;
+    const double* un;
+    double* out;
+    int tix, tiy, tiz;
+};
+
+__device__ __forceinline__ TileRef resolve_tile(const StageArgs& A, int t) {
+    TileRef r;
+    int tile0, tiles0, tiles1;
+    const double *b0, *b1, *b2;
+    if (A.batch.table) {
+        const BatchBlock& BB = A.batch.table[A.batch.tile_block[t]];
+        r.b = BB.b;
+        b0 = BB.buf[0];
+        b1 = BB.buf[1];
+        b2 = BB.buf[2];
+        tile0 = BB.tile0;
+        tiles0 = BB.tiles[0];
+        tiles1 = BB.tiles[1];
+    } else {
+        r.b = A.batch.one.b;
+        b0 = A.batch.one.buf[0];
+        b1 = A.batch.one.buf[1];
+        b2 = A.batch.one.buf[2];
+        tile0 = 0;
+        tiles0 = A.batch.one.tiles[0];
+        tiles1 = A.batch.one.tiles[1];
+    }
+    const double* bufs[3] = {b0, b1, b2};
+    r.in = bufs[A.sel_in];
+    r.un = bufs[A.sel_un];
+    r.out = const_cast<double*>(bufs[A.sel_out]);
+    const int lt = t - tile0;
+    r.tix = lt % tiles0;
+    r.tiy = (lt / tiles0) % tiles1;
+    r.tiz = lt / (tiles0 * tiles1);
+    return r;
+}
+
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
     using T = TileGeom<TX, TY, TZ>;
@@ -413,28 +457,27 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(St
 
     const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
     if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
-    const int tix = tile % A.tiles[0];
-    const int tiy = (tile / A.tiles[0]) % A.tiles[1];
-    const int tiz = tile / (A.tiles[0] * A.tiles[1]);
+    const TileRef R = resolve_tile(A, tile);
+    const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
-    const int ix = tix * TX + lx, iy = tiy * TY + ly, iz = tiz * TZ + lz; // interior-relative
-    const bool active = ix < A.n[0] && iy < A.n[1] && iz < A.n[2];
-    const int gi = A.ng + (active ? ix : 0), gj = A.ng + (active ? iy : 0), gk = A.ng + (active ? iz : 0);
-    const long cidx = gi + (long)A.nt[0] * (gj + (long)A.nt[1] * gk);
+    const int ix = R.tix * TX + lx, iy = R.tiy * TY + ly, iz = R.tiz * TZ + lz; // interior-relative
+    const bool active = ix < b.n[0] && iy < b.n[1] && iz < b.n[2];
+    const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
+    const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
 
     TileIO<TX, TY, TZ, MODE, GR_FP64_PREFETCH> io;
     io.lds = lds;
-    io.in = A.in;
-    io.un = A.un;
-    io.out = A.out;
-    io.stride = A.stride;
+    io.in = R.in;
+    io.un = R.un;
+    io.out = R.out;
+    io.stride = b.stride;
     io.cidx = cidx;
     io.cl = T::lds(lx, ly, lz);
     io.active = active;
     io.ko_on = A.ko_on != 0;
     io.cf = A.cf;
-    io.init(tid, A.ng + tix * TX, A.ng + tiy * TY, A.ng + tiz * TZ, A.nt[0], A.nt[1], A.nt[2]);
+    io.init(tid, b.ng + R.tix * TX, b.ng + R.tiy * TY, b.ng + R.tiz * TZ, b.nt[0], b.nt[1], b.nt[2]);
 
     CellSources src{};
     if (HAS_SRC) {
@@ -452,34 +495,33 @@ constexpr int FP64_TX = GR_FP64_TILE_X, FP64_TY = GR_FP64_TILE_Y, FP64_TZ = GR_F
 
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC>
 __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_mixed(StageArgs A) {
-    using T = TileGeom<TX, TY, TZ>;
+    using T = TileGeom<TX, TY, TZ, false>;
     __shared__ float lds[2 * (T::NLDS + 1)];
 
     const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
     if (A.flags && A.flags[tile] != A.want) return;
-    const int tix = tile % A.tiles[0];
-    const int tiy = (tile / A.tiles[0]) % A.tiles[1];
-    const int tiz = tile / (A.tiles[0] * A.tiles[1]);
+    const TileRef R = resolve_tile(A, tile);
+    const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
-    const int ix = tix * TX + lx, iy = tiy * TY + ly, iz = tiz * TZ + lz;
-    const bool active = ix < A.n[0] && iy < A.n[1] && iz < A.n[2];
-    const int gi = A.ng + (active ? ix : 0), gj = A.ng + (active ? iy : 0), gk = A.ng + (active ? iz : 0);
-    const long cidx = gi + (long)A.nt[0] * (gj + (long)A.nt[1] * gk);
+    const int ix = R.tix * TX + lx, iy = R.tiy * TY + ly, iz = R.tiz * TZ + lz;
+    const bool active = ix < b.n[0] && iy < b.n[1] && iz < b.n[2];
+    const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
+    const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
 
     TileIO32<TX, TY, TZ, MODE, GR_MIXED_PREFETCH> io;
     io.lds = lds;
-    io.in = A.in;
-    io.un = A.un;
-    io.out = A.out;
-    io.stride = A.stride;
+    io.in = R.in;
+    io.un = R.un;
+    io.out = R.out;
+    io.stride = b.stride;
     io.cidx = cidx;
-    io.ridx = (A.ng + tix * TX) + (long)A.nt[0] * ((A.ng + tiy * TY) + (long)A.nt[1] * (A.ng + tiz * TZ));
+    io.ridx = (b.ng + R.tix * TX) + (long)b.nt[0] * ((b.ng + R.tiy * TY) + (long)b.nt[1] * (b.ng + R.tiz * TZ));
     io.cl = T::lds(lx, ly, lz);
     io.active = active;
     io.ko_on = A.ko_on != 0;
     io.cf = A.cf;
-    io.init(tid, A.ng + tix * TX, A.ng + tiy * TY, A.ng + tiz * TZ, A.nt[0], A.nt[1], A.nt[2]);
+    io.init(tid, b.ng + R.tix * TX, b.ng + R.tiy * TY, b.ng + R.tiz * TZ, b.nt[0], b.nt[1], b.nt[2]);
 
     CellSources src{};
     if (HAS_SRC) {
@@ -496,17 +538,18 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_m
 // flags[tile] = 1 where the tile needs the FP64 kernel in the mixed form: its smallest chi
 // (interior cells of the stage input) is below chi_min or not finite.
 template <int TX, int TY, int TZ>
-__global__ __launch_bounds__(TX* TY* TZ) void classify_tiles_kernel(BlockDesc b, const double* u, int t0, int t1,
-                                                                      double chi_min, unsigned char* flags) {
+__global__ __launch_bounds__(TX* TY* TZ) void classify_tiles_kernel(StageArgs A, double chi_min, unsigned char* flags) {
     __shared__ int bad;
     const int tile = blockIdx.x;
-    const int tix = tile % t0, tiy = (tile / t0) % t1, tiz = tile / (t0 * t1);
+    const TileRef R = resolve_tile(A, tile);
+    const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
-    const int ix = tix * TX + tid % TX, iy = tiy * TY + (tid / TX) % TY, iz = tiz * TZ + tid / (TX * TY);
+    const int ix = R.tix * TX + tid % TX, iy = R.tiy * TY + (tid / TX) % TY, iz = R.tiz * TZ + tid / (TX * TY);
     if (tid == 0) bad = 0;
     __syncthreads();
     if (ix < b.n[0] && iy < b.n[1] && iz < b.n[2]) {
-        const double c = u[CHI * b.stride + (b.ng + ix) + (long)b.nt[0] * ((b.ng + iy) + (long)b.nt[1] * (b.ng + iz))];
+        const double c =
+            R.in[CHI * b.stride + (b.ng + ix) + (long)b.nt[0] * ((b.ng + iy) + (long)b.nt[1] * (b.ng + iz))];
         if (!(c >= chi_min)) bad = 1; // benign race: every writer stores 1
     }
     __syncthreads();
@@ -543,13 +586,15 @@ __device__ __forceinline__ double f_inf(int v) {
 // so a cell that is a ghost along a set of axes ends with the chain of 1/r fall-offs from
 // its interior anchor, applied in X, Y, Z order at the intermediate positions
 // (src/main.cpp:395-483).
-__global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode) {
+__device__ __forceinline__ long ghost_count(const BlockDesc& b) {
+    return 2L * b.ng * b.nt[1] * b.nt[0] + (long)b.n[2] * 2 * b.ng * b.nt[0] + (long)b.n[2] * b.n[1] * 2 * b.ng;
+}
+
+// ghost cell number t of block b (z slabs, then y rows of interior z, then x columns)
+__device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t, int mode) {
 #pragma clang fp contract(off)
     const long nA = 2L * b.ng * b.nt[1] * b.nt[0];
     const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
-    const long nC = (long)b.n[2] * b.n[1] * 2 * b.ng;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nA + nB + nC) return;
     const int ng = b.ng, g2 = 2 * ng;
     int i, j, k;
     if (t < nA) {
@@ -611,6 +656,27 @@ __global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode) {
         for (int s = 0; s < nsteps; ++s) val = fi + (val - fi) * rin[s] / rgh[s];
         u[v * b.stride + dst] = val;
     }
+}
+
+__global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ghost_count(b)) return;
+    fill_ghost_cell(b, u, t, mode);
+}
+
+// every ghost cell of buffer `sel` of every block of a table; ofs[i] = first ghost cell of
+// block i (nblocks + 1 entries)
+__global__ void fill_ghosts_batch_kernel(const BatchBlock* table, const long* ofs, int nblocks, int sel, int mode) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ofs[nblocks]) return;
+    int lo = 0, hi = nblocks - 1; // largest i with ofs[i] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ofs[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchBlock& B = table[lo];
+    fill_ghost_cell(B.b, B.buf[sel], t - ofs[lo], mode);
 }
 
 // ------------------------------------------------------------------------------------
@@ -930,26 +996,19 @@ void stage_tile_fp64(int* t3) {
 #define GR_XCD_REMAP 1
 #endif
 
-static StageArgs stage_args(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
-                            const CellParams& P, double ko_sigma, const StageCoef& cf, const unsigned char* flags,
-                            int want) {
+static StageArgs stage_args(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+                            double ko_sigma, const StageCoef& cf, const unsigned char* flags, int want) {
     StageArgs A;
-    A.in = in;
-    A.un = un;
-    A.out = out;
+    A.batch = B;
+    A.sel_in = sel[0];
+    A.sel_un = sel[1];
+    A.sel_out = sel[2];
     A.src = src;
-    for (int d = 0; d < 3; ++d) {
-        A.n[d] = b.n[d];
-        A.nt[d] = b.nt[d];
-    }
-    A.stride = b.stride;
-    A.ng = b.ng;
-    A.tiles[0] = (b.n[0] + FP64_TX - 1) / FP64_TX;
-    A.tiles[1] = (b.n[1] + FP64_TY - 1) / FP64_TY;
-    A.tiles[2] = (b.n[2] + FP64_TZ - 1) / FP64_TZ;
-    A.ntiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+    A.ntiles = B.ntiles;
     A.xcd_remap = (GR_XCD_REMAP && A.ntiles % 8 == 0) ? 1 : 0;
-    A.K = make_stencil(b, ko_sigma);
+    BlockDesc g;
+    for (int d = 0; d < 3; ++d) g.dx[d] = B.dx[d];
+    A.K = make_stencil(g, ko_sigma);
     for (int d = 0; d < 3; ++d) {
         A.K32.ih[d] = (float)A.K.ih[d];
         A.K32.ih2[d] = (float)A.K.ih2[d];
@@ -970,10 +1029,54 @@ int stage_tiles(const BlockDesc& b) {
         ((b.n[2] + FP64_TZ - 1) / FP64_TZ);
 }
 
-hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
-                             const CellParams& P, double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
+static long host_ghost_count(const BlockDesc& b) {
+    return 2L * b.ng * b.nt[1] * b.nt[0] + (long)b.n[2] * 2 * b.ng * b.nt[0] + (long)b.n[2] * b.n[1] * 2 * b.ng;
+}
+
+int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, std::vector<long>& ghost_ofs) {
+    int t = 0;
+    long g = 0;
+    tile_block.clear();
+    ghost_ofs.assign(1, 0);
+    for (size_t i = 0; i < blocks.size(); ++i) {
+        BatchBlock& B = blocks[i];
+        B.tiles[0] = (B.b.n[0] + FP64_TX - 1) / FP64_TX;
+        B.tiles[1] = (B.b.n[1] + FP64_TY - 1) / FP64_TY;
+        B.tiles[2] = (B.b.n[2] + FP64_TZ - 1) / FP64_TZ;
+        B.tile0 = t;
+        const int nt = B.tiles[0] * B.tiles[1] * B.tiles[2];
+        tile_block.insert(tile_block.end(), nt, (int)i);
+        t += nt;
+        g += host_ghost_count(B.b);
+        ghost_ofs.push_back(g);
+    }
+    return t;
+}
+
+Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out) {
+    Batch B;
+    B.table = nullptr;
+    B.tile_block = nullptr;
+    B.ghost_ofs = nullptr;
+    B.nblocks = 1;
+    B.one.b = b;
+    B.one.buf[0] = const_cast<double*>(in);
+    B.one.buf[1] = const_cast<double*>(un);
+    B.one.buf[2] = out;
+    B.one.tile0 = 0;
+    B.one.tiles[0] = (b.n[0] + FP64_TX - 1) / FP64_TX;
+    B.one.tiles[1] = (b.n[1] + FP64_TY - 1) / FP64_TY;
+    B.one.tiles[2] = (b.n[2] + FP64_TZ - 1) / FP64_TZ;
+    B.ntiles = B.one.tiles[0] * B.one.tiles[1] * B.one.tiles[2];
+    B.nghost = host_ghost_count(b);
+    for (int d = 0; d < 3; ++d) B.dx[d] = b.dx[d];
+    return B;
+}
+
+hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+                             double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags) {
-    const StageArgs A = stage_args(b, in, un, out, src, P, ko_sigma, cf, flags, 1);
+    const StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 1);
     const bool has_src = src.rho != nullptr;
     if (has_src) {
         if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, true>(A, A.ntiles, s);
@@ -992,17 +1095,15 @@ static hipError_t launch_mixed_t(const StageArgs& A, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_stage_mixed(const BlockDesc& b, const double* in, const double* un, double* out,
-                              const Sources& src, const CellParams& P, double ko_sigma, int mode,
-                              const StageCoef& cf, hipStream_t s, unsigned char* flags, double chi_fp64_below) {
-    const int nt = stage_tiles(b);
-    const int t0 = (b.n[0] + FP64_TX - 1) / FP64_TX, t1 = (b.n[1] + FP64_TY - 1) / FP64_TY;
+hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s, unsigned char* flags,
+                              double chi_fp64_below) {
+    const StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 0);
     classify_tiles_kernel<FP64_TX, FP64_TY, FP64_TZ>
-        <<<dim3(nt), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(b, in, t0, t1, chi_fp64_below, flags);
+        <<<dim3(A.ntiles), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A, chi_fp64_below, flags);
     hipError_t st = hipGetLastError();
     if (st != hipSuccess) return st;
     // FP32 tiles (flag 0)
-    const StageArgs A = stage_args(b, in, un, out, src, P, ko_sigma, cf, flags, 0);
     const bool has_src = src.rho != nullptr;
     if (has_src) {
         if (mode == MODE_RHS) st = launch_mixed_t<MODE_RHS, true>(A, s);
@@ -1015,7 +1116,13 @@ hipError_t launch_stage_mixed(const BlockDesc& b, const double* in, const double
     }
     if (st != hipSuccess) return st;
     // FP64 tiles near the puncture (flag 1)
-    return launch_stage_fp64(b, in, un, out, src, P, ko_sigma, mode, cf, s, flags);
+    return launch_stage_fp64(B, sel, src, P, ko_sigma, mode, cf, s, flags);
+}
+
+hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream_t s) {
+    if (!B.table) return launch_fill_ghosts(B.one.b, B.one.buf[sel], mode, s);
+    fill_ghosts_batch_kernel<<<grid_for(B.nghost, 256), 256, 0, s>>>(B.table, B.ghost_ofs, B.nblocks, sel, mode);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s) {

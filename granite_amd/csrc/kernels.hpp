@@ -6,6 +6,8 @@
 
 #include "ccz4_cell.hpp"
 
+#include <vector>
+
 namespace gr {
 
 // one block as the kernels see it
@@ -19,6 +21,37 @@ struct BlockDesc {
 };
 
 enum StageMode : int { MODE_RHS = 0, MODE_STAGE = 1, MODE_FINAL = 2 };
+
+// One block of a batched launch: its geometry, its three state buffers (state u and the
+// two RK3 stage buffers, or for a single-block call: input, u^n, output) and the index of
+// its first stage tile in the launch.
+struct BatchBlock {
+    BlockDesc b;
+    double* buf[3];
+    int tile0;
+    int tiles[3];
+};
+
+// What a stage / ghost launch runs over: a device table of blocks (table != nullptr,
+// with tile_block[t] = block of global tile t and ghost_ofs[i] = first ghost cell of
+// block i, nblocks + 1 entries) or the single block `one`. sel_* pick the buffer of
+// each block that is the stage input, u^n and the output (0, 1, 2 = buf[0..2]).
+struct Batch {
+    const BatchBlock* table;
+    const int* tile_block;
+    const long* ghost_ofs;
+    int nblocks;
+    BatchBlock one;
+    int ntiles;
+    long nghost;
+    double dx[3]; // grid spacing shared by all blocks of the launch
+};
+
+// builds the tile bookkeeping of a block list (host side): tile0/tiles of each block,
+// the tile->block map and the ghost-cell prefix. Returns the total number of tiles.
+int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, std::vector<long>& ghost_ofs);
+// a one-block batch over (in, un, out)
+Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out);
 
 struct StageCoef {
     double a1;   // weight of u^n (read pointwise from `un`)
@@ -36,20 +69,24 @@ struct Sources {
 // stencil constants for a block, with the reference's operation order
 Stencil<double> make_stencil(const BlockDesc& b, double ko_sigma);
 
-// RHS / RK3 stage kernel on one block (FP64). in: stencil input (ghosts filled);
-// un: u^n for the combine (ignored for MODE_RHS); out: RHS (MODE_RHS) or the new state.
-hipError_t launch_stage_fp64(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src,
-                             const CellParams& P, double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
+// RHS / RK3 stage kernel (FP64) over every tile of a batch. sel = {input, u^n, output}
+// buffer indices; input ghosts must be filled; u^n is ignored for MODE_RHS; the output
+// receives the RHS (MODE_RHS) or the new stage state. Sources apply to single-block
+// batches only.
+hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+                             double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags = nullptr);
 
-// Mixed-precision stage: classifies the tiles of `in` (FP64 where the smallest chi of the
-// tile is below chi_fp64_below), then runs the FP32 kernel on the other tiles and the FP64
-// kernel on those. flags: device buffer of stage_tiles(b) bytes.
-hipError_t launch_stage_mixed(const BlockDesc& b, const double* in, const double* un, double* out,
-                              const Sources& src, const CellParams& P, double ko_sigma, int mode,
-                              const StageCoef& cf, hipStream_t s, unsigned char* flags, double chi_fp64_below);
+// Mixed-precision stage: classifies the tiles of the input (FP64 where the smallest chi of
+// the tile is below chi_fp64_below), then runs the FP32 kernel on the other tiles and the
+// FP64 kernel on those. flags: device buffer of B.ntiles bytes.
+hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s, unsigned char* flags,
+                              double chi_fp64_below);
 // number of stage tiles of a block
 int stage_tiles(const BlockDesc& b);
+// ghost fill of buffer `sel` of every block of a batch (mode as launch_fill_ghosts)
+hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream_t s);
 
 // Ghost fill of one block: mode 0 Sommerfeld (chained X,Y,Z), 1 periodic, 2 outflow copy.
 hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s);
