@@ -19,6 +19,7 @@ from .ccz4 import (  # noqa: F401
     Engine,
     EngineError,
     GridBlock,
+    Hierarchy,
     Level,
     SpacetimeVar,
     library,

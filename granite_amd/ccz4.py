@@ -117,6 +117,13 @@ def library():
         "gr_level_set_mixed_policy": (C.c_int, [vp, C.c_double]),
         "gr_level_rhs": (C.c_int, [vp, C.c_int, C.POINTER(_Params), C.c_int, _dp]),
         "gr_level_profile_read": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_int)]),
+        "gr_hierarchy_create": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(vp)]),
+        "gr_hierarchy_destroy": (C.c_int, [vp]),
+        "gr_hierarchy_prolongate": (C.c_int, [vp, C.c_int]),
+        "gr_hierarchy_restrict": (C.c_int, [vp, C.c_int]),
+        "gr_hierarchy_fill_ghosts": (C.c_int, [vp, C.c_int]),
+        "gr_hierarchy_step": (C.c_int, [vp, C.POINTER(_Params), C.c_double, C.c_int, C.c_int]),
+        "gr_hierarchy_constraints_l2": (C.c_int, [vp, _dp, _dp, C.POINTER(C.c_longlong)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -407,6 +414,57 @@ class Level:
     def close(self):
         if getattr(self, "_h", None):
             self._e.lib.gr_level_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Hierarchy:
+    """Levels of a Berger-Oliger hierarchy on the device (gr_hierarchy), coarsest first.
+
+    Plays amr::AMRHierarchy for a fixed geometry (include/granite/amr/amr.hpp:49-117):
+    fill_ghosts = fillGhostZones, prolongate / restrict = the inter-level transfers of a
+    whole level, step = setLevelDt(0, dt0) + subcycle(0) with the SSP-RK3 level step."""
+
+    def __init__(self, levels, engine: Engine | None = None):
+        self._e = engine if engine is not None else Engine.default()
+        self.levels = list(levels)
+        arr = (C.c_void_p * len(self.levels))(*[lv._h for lv in self.levels])
+        h = C.c_void_p()
+        self._e.check(self._e.lib.gr_hierarchy_create(self._e.handle, len(self.levels), arr, C.byref(h)))
+        self._h = h
+
+    def prolongate(self, level):
+        self._e.check(self._e.lib.gr_hierarchy_prolongate(self._h, int(level)))
+
+    def restrict(self, level):
+        self._e.check(self._e.lib.gr_hierarchy_restrict(self._h, int(level)))
+
+    def fill_ghosts(self, level):
+        self._e.check(self._e.lib.gr_hierarchy_fill_ghosts(self._h, int(level)))
+
+    def step(self, params: CCZ4Params, dt0, precision=PREC_FP64, use_graph=False):
+        self._e.check(self._e.lib.gr_hierarchy_step(self._h, C.byref(params.c()), float(dt0), int(precision),
+                                                    1 if use_graph else 0))
+
+    def constraints_l2(self):
+        """(sum H^2, sum |M|^2, count) over the interiors of every block of every level."""
+        h = C.c_double()
+        m = C.c_double()
+        n = C.c_longlong()
+        self._e.check(self._e.lib.gr_hierarchy_constraints_l2(self._h, C.byref(h), C.byref(m), C.byref(n)))
+        return h.value, m.value, n.value
+
+    def sync(self):
+        self._e.check(self._e.lib.gr_engine_sync(self._e.handle))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._e.lib.gr_hierarchy_destroy(self._h)
             self._h = None
 
     def __del__(self):

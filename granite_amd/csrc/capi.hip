@@ -35,6 +35,8 @@ struct gr_level {
     std::vector<gr::BlockDesc> blk;
     std::vector<double*> u, s1, s2;
     double* ham = nullptr;     // monitor scratch (max stride)
+    double* mom = nullptr;     // momentum-constraint scratch (3 * max stride, on first use)
+    long maxstride = 0;
     double* partial = nullptr; // reduction partials
     double* red = nullptr;     // reduction result (1 double)
     double* bh = nullptr;      // puncture masses/positions for initial data
@@ -324,6 +326,7 @@ int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** 
     std::vector<long> ghost_ofs;
     const int ntiles = gr::batch_layout(L->table, tile_block, ghost_ofs);
     hipError_t st = hipMalloc((void**)&L->ham, maxstride * sizeof(double));
+    L->maxstride = maxstride;
     if (st == hipSuccess) st = hipMalloc((void**)&L->partial, L->partial_cap * sizeof(double));
     if (st == hipSuccess) st = hipMalloc((void**)&L->red, sizeof(double));
     if (st == hipSuccess) st = hipMalloc((void**)&L->flags, ntiles);
@@ -379,6 +382,7 @@ int gr_level_destroy(gr_level* L) {
     if (L->d_tile_block) (void)hipFree(L->d_tile_block);
     if (L->d_ghost_ofs) (void)hipFree(L->d_ghost_ofs);
     if (L->ham) (void)hipFree(L->ham);
+    if (L->mom) (void)hipFree(L->mom);
     if (L->partial) (void)hipFree(L->partial);
     if (L->red) (void)hipFree(L->red);
     if (L->bh) (void)hipFree(L->bh);
@@ -629,7 +633,7 @@ int gr_level_constraints_l2(gr_level* L, double* sumsq, long long* count) {
         const gr::BlockDesc& B = L->blk[b];
         int nparts = 0;
         GR_HIP(e, gr::launch_constraints(B, L->u[b], L->ham, nullptr, e->stream));
-        GR_HIP(e, gr::launch_sumsq_interior(B, L->ham, L->partial, L->partial_cap, &nparts, e->stream));
+        GR_HIP(e, gr::launch_sumsq_interior(B, L->ham, 1, L->partial, L->partial_cap, &nparts, e->stream));
         GR_HIP(e, gr::launch_sum_partials(L->partial, nparts, L->red, e->stream));
         double h = 0.0;
         GR_HIP(e, hipMemcpyAsync(&h, L->red, sizeof(double), hipMemcpyDeviceToHost, e->stream));
@@ -659,6 +663,240 @@ int gr_level_nonfinite(gr_level* L, long long* count) {
         total += h;
     }
     *count = (long long)total;
+    return GR_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Berger-Oliger hierarchy
+// ---------------------------------------------------------------------------------------
+
+struct gr_hierarchy {
+    gr_engine* eng = nullptr;
+    std::vector<gr_level*> lv;
+    // per level, per block: outflow faces (bit 2d low, 2d+1 high side of axis d) that no
+    // block of the parent level covers; all six on level 0
+    std::vector<std::vector<int>> faces;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool have_graph = false;
+    double gkey[8] = {0};
+    std::vector<double> policy; // each level's mixed-precision threshold at capture
+};
+
+namespace {
+
+// GridBlock::x (include/granite/core/grid.hpp:92-94)
+double cell_x(const gr::BlockDesc& b, int d, int idx) { return b.lo[d] + ((double)idx - b.ng + 0.5) * b.dx[d]; }
+
+// faces of fine block f left to the outflow copy by fillGhostZones (src/amr/amr.cpp:412-428):
+// a side counts as covered when any parent block's first (last) interior cell centre lies
+// at or below (above) the fine block's first (last) interior centre -/+ half a fine cell
+int uncovered_faces(const gr::BlockDesc& f, const std::vector<gr::BlockDesc>& parents) {
+    bool lo[3] = {false, false, false}, hi[3] = {false, false, false};
+    for (const gr::BlockDesc& c : parents)
+        for (int d = 0; d < 3; ++d) {
+            const double fine_lo = cell_x(f, d, f.ng), fine_hi = cell_x(f, d, f.ng + f.n[d] - 1);
+            const double c_lo = cell_x(c, d, c.ng), c_hi = cell_x(c, d, c.ng + c.n[d] - 1);
+            if (c_lo <= fine_lo + 0.5 * f.dx[d]) lo[d] = true;
+            if (c_hi >= fine_hi - 0.5 * f.dx[d]) hi[d] = true;
+        }
+    int m = 0;
+    for (int d = 0; d < 3; ++d) m |= (lo[d] ? 0 : 1 << (2 * d)) | (hi[d] ? 0 : 1 << (2 * d + 1));
+    return m;
+}
+
+hipError_t h_prolongate(gr_hierarchy* H, int l) {
+    gr_level* F = H->lv[l];
+    gr_level* C = H->lv[l - 1];
+    for (size_t f = 0; f < F->blk.size(); ++f)
+        for (size_t c = 0; c < C->blk.size(); ++c) {
+            hipError_t st = gr::launch_prolongate(C->blk[c], C->u[c], F->blk[f], F->u[f], H->eng->stream);
+            if (st != hipSuccess) return st;
+        }
+    return hipSuccess;
+}
+
+hipError_t h_restrict(gr_hierarchy* H, int l) {
+    gr_level* F = H->lv[l];
+    gr_level* C = H->lv[l - 1];
+    for (size_t f = 0; f < F->blk.size(); ++f)
+        for (size_t c = 0; c < C->blk.size(); ++c) {
+            hipError_t st = gr::launch_restrict(F->blk[f], F->u[f], C->blk[c], C->u[c], H->eng->stream);
+            if (st != hipSuccess) return st;
+        }
+    return hipSuccess;
+}
+
+// fillGhostZones(l), src/amr/amr.cpp:400-482
+hipError_t h_fill(gr_hierarchy* H, int l) {
+    gr_level* L = H->lv[l];
+    hipStream_t s = H->eng->stream;
+    if (l == 0) return gr::launch_fill_ghosts_batch(L->batch, 0, GR_BC_OUTFLOW, s);
+    hipError_t st = h_prolongate(H, l);
+    for (size_t b = 0; st == hipSuccess && b < L->blk.size(); ++b)
+        if (H->faces[l][b]) st = gr::launch_fill_ghosts(L->blk[b], L->u[b], GR_BC_OUTFLOW, s, H->faces[l][b]);
+    return st;
+}
+
+// subcycle(l), src/amr/amr.cpp:303-350, without the regrid of its step 6. Its step 1
+// (prolongation into level l > 0) is the first half of fillGhostZones(l), which follows
+// at once and reads the same parent data, so the prolongation runs once.
+hipError_t h_subcycle(gr_hierarchy* H, int l, double dt, const gr_ccz4_params* prm, int precision) {
+    hipError_t st = h_fill(H, l);
+    if (st == hipSuccess) st = enqueue_rk3(H->lv[l], prm, dt, GR_BC_SOMMERFELD, GR_BC_NONE, precision);
+    if (st == hipSuccess) st = h_fill(H, l);
+    if (st == hipSuccess && l + 1 < (int)H->lv.size()) {
+        const double dt_fine = dt / 2.0;
+        for (int r = 0; r < 2 && st == hipSuccess; ++r) st = h_subcycle(H, l + 1, dt_fine, prm, precision);
+        if (st == hipSuccess) st = h_restrict(H, l + 1);
+    }
+    return st;
+}
+
+int check_hier_level(gr_hierarchy* H, int level, int lo) {
+    if (!H) return GR_ERR_ARG;
+    if (level < lo || level >= (int)H->lv.size()) return fail(H->eng, GR_ERR_ARG, "level index out of range");
+    return use_device(H->eng);
+}
+
+} // namespace
+
+extern "C" {
+
+int gr_hierarchy_create(gr_engine* e, int nlevels, gr_level* const* levels, gr_hierarchy** out) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!out || !levels || nlevels < 1) return fail(e, GR_ERR_ARG, "bad hierarchy arguments");
+    *out = nullptr;
+    for (int l = 0; l < nlevels; ++l) {
+        if (!levels[l] || levels[l]->eng != e) return fail(e, GR_ERR_ARG, "every level must belong to this engine");
+        if (l > 0)
+            for (int d = 0; d < 3; ++d) {
+                const double want = levels[l - 1]->blk[0].dx[d] / 2.0;
+                if (std::fabs(levels[l]->blk[0].dx[d] - want) > 1e-6 * want)
+                    return fail(e, GR_ERR_GEOM, "dx of each level must be half of its parent's (refinement ratio 2)");
+            }
+    }
+    gr_hierarchy* H = new gr_hierarchy();
+    H->eng = e;
+    H->lv.assign(levels, levels + nlevels);
+    H->faces.resize(nlevels);
+    H->faces[0].assign(levels[0]->blk.size(), 0x3f);
+    for (int l = 1; l < nlevels; ++l)
+        for (const gr::BlockDesc& f : levels[l]->blk) H->faces[l].push_back(uncovered_faces(f, levels[l - 1]->blk));
+    *out = H;
+    return GR_OK;
+}
+
+int gr_hierarchy_destroy(gr_hierarchy* H) {
+    if (!H) return GR_ERR_ARG;
+    (void)hipSetDevice(H->eng->device);
+    (void)hipStreamSynchronize(H->eng->stream);
+    if (H->exec) (void)hipGraphExecDestroy(H->exec);
+    if (H->graph) (void)hipGraphDestroy(H->graph);
+    delete H;
+    return GR_OK;
+}
+
+int gr_hierarchy_prolongate(gr_hierarchy* H, int level) {
+    int rc = check_hier_level(H, level, 1);
+    if (rc) return rc;
+    GR_HIP(H->eng, h_prolongate(H, level));
+    return GR_OK;
+}
+
+int gr_hierarchy_restrict(gr_hierarchy* H, int level) {
+    int rc = check_hier_level(H, level, 1);
+    if (rc) return rc;
+    GR_HIP(H->eng, h_restrict(H, level));
+    return GR_OK;
+}
+
+int gr_hierarchy_fill_ghosts(gr_hierarchy* H, int level) {
+    int rc = check_hier_level(H, level, 0);
+    if (rc) return rc;
+    GR_HIP(H->eng, h_fill(H, level));
+    return GR_OK;
+}
+
+int gr_hierarchy_step(gr_hierarchy* H, const gr_ccz4_params* prm, double dt0, int precision, int use_graph) {
+    if (!H) return GR_ERR_ARG;
+    gr_engine* e = H->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if ((rc = check_params(e, prm))) return rc;
+    if (!std::isfinite(dt0)) return fail(e, GR_ERR_ARG, "dt must be finite");
+    if (precision != GR_PREC_FP64 && precision != GR_PREC_MIXED) return fail(e, GR_ERR_ARG, "unknown precision");
+    bool profiling = false;
+    for (gr_level* L : H->lv) profiling = profiling || L->profile;
+    if (!use_graph || profiling) {
+        GR_HIP(e, h_subcycle(H, 0, dt0, prm, precision));
+        return GR_OK;
+    }
+    const double key[8] = {dt0, prm->kappa1, prm->kappa2, prm->eta, prm->ko_sigma, prm->chi_blend_center,
+                           prm->chi_blend_width, (double)precision};
+    // the mixed-precision threshold of every level is baked into the captured kernels
+    std::vector<double> policy;
+    for (gr_level* L : H->lv) policy.push_back(L->chi_fp64_below);
+    const bool fresh = H->have_graph && std::memcmp(key, H->gkey, sizeof(key)) == 0 && policy == H->policy;
+    if (!fresh) {
+        if (H->exec) GR_HIP(e, hipGraphExecDestroy(H->exec));
+        if (H->graph) GR_HIP(e, hipGraphDestroy(H->graph));
+        H->exec = nullptr;
+        H->graph = nullptr;
+        H->have_graph = false;
+        GR_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t st = h_subcycle(H, 0, dt0, prm, precision);
+        hipGraph_t g = nullptr;
+        hipError_t st2 = hipStreamEndCapture(e->stream, &g);
+        if (st != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hipfail(e, st, "capture of the coarse step");
+        }
+        if (st2 != hipSuccess) return hipfail(e, st2, "hipStreamEndCapture");
+        H->graph = g;
+        GR_HIP(e, hipGraphInstantiate(&H->exec, H->graph, nullptr, nullptr, 0));
+        std::memcpy(H->gkey, key, sizeof(key));
+        H->policy = policy;
+        H->have_graph = true;
+    }
+    GR_HIP(e, hipGraphLaunch(H->exec, e->stream));
+    return GR_OK;
+}
+
+int gr_hierarchy_constraints_l2(gr_hierarchy* H, double* ham_sumsq, double* mom_sumsq, long long* count) {
+    if (!H) return GR_ERR_ARG;
+    gr_engine* e = H->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!count) return fail(e, GR_ERR_ARG, "null output");
+    double th = 0.0, tm = 0.0;
+    long long n = 0;
+    for (gr_level* L : H->lv) {
+        if (mom_sumsq && !L->mom) GR_HIP(e, hipMalloc((void**)&L->mom, 3 * L->maxstride * sizeof(double)));
+        for (size_t b = 0; b < L->blk.size(); ++b) {
+            const gr::BlockDesc& B = L->blk[b];
+            GR_HIP(e, gr::launch_constraints(B, L->u[b], L->ham, mom_sumsq ? L->mom : nullptr, e->stream));
+            double h[2] = {0.0, 0.0};
+            for (int q = 0; q < 2; ++q) {
+                if (q == 0 ? !ham_sumsq : !mom_sumsq) continue;
+                int nparts = 0;
+                GR_HIP(e, gr::launch_sumsq_interior(B, q == 0 ? L->ham : L->mom, q == 0 ? 1 : 3, L->partial,
+                                                    L->partial_cap, &nparts, e->stream));
+                GR_HIP(e, gr::launch_sum_partials(L->partial, nparts, L->red, e->stream));
+                GR_HIP(e, hipMemcpyAsync(&h[q], L->red, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+                GR_HIP(e, hipStreamSynchronize(e->stream));
+            }
+            th += h[0];
+            tm += h[1];
+            n += (long long)B.n[0] * B.n[1] * B.n[2];
+        }
+    }
+    if (ham_sumsq) *ham_sumsq = th;
+    if (mom_sumsq) *mom_sumsq = tm;
+    *count = n;
     return GR_OK;
 }
 

@@ -590,8 +590,12 @@ __device__ __forceinline__ long ghost_count(const BlockDesc& b) {
     return 2L * b.ng * b.nt[1] * b.nt[0] + (long)b.n[2] * 2 * b.ng * b.nt[0] + (long)b.n[2] * b.n[1] * 2 * b.ng;
 }
 
-// ghost cell number t of block b (z slabs, then y rows of interior z, then x columns)
-__device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t, int mode) {
+// ghost cell number t of block b (z slabs, then y rows of interior z, then x columns).
+// For the outflow copy (mode 2) only the faces whose bit is set in `faces` are filled
+// (bit 2d: low side of axis d, bit 2d+1: high side); the others keep their values, as
+// fillGhostZones leaves the faces covered by a parent level to prolongation
+// (src/amr/amr.cpp:412-480).
+__device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t, int mode, int faces = 0x3f) {
 #pragma clang fp contract(off)
     const long nA = 2L * b.ng * b.nt[1] * b.nt[0];
     const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
@@ -626,7 +630,9 @@ __device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t, int mode)
             if (r < 0) r += b.n[d];
             A[d] = ng + r;
         } else {
-            A[d] = I[d] < ng ? ng : (I[d] >= ng + b.n[d] ? ng + b.n[d] - 1 : I[d]);
+            const bool lo = I[d] < ng, hi = I[d] >= ng + b.n[d];
+            const bool fill = mode != 2 || (lo && ((faces >> (2 * d)) & 1)) || (hi && ((faces >> (2 * d + 1)) & 1));
+            A[d] = (lo && fill) ? ng : ((hi && fill) ? ng + b.n[d] - 1 : I[d]);
         }
     }
     const long dst = I[0] + (long)b.nt[0] * (I[1] + (long)b.nt[1] * I[2]);
@@ -658,10 +664,82 @@ __device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t, int mode)
     }
 }
 
-__global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode) {
+__global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode, int faces) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ghost_count(b)) return;
-    fill_ghost_cell(b, u, t, mode);
+    fill_ghost_cell(b, u, t, mode, faces);
+}
+
+// ------------------------------------------------------------------------------------
+// AMR transfer between a coarse and a fine block (one block per level)
+// ------------------------------------------------------------------------------------
+
+// Trilinear coarse -> fine into every fine cell, ghosts included, with the reference's
+// index rounding and clamping (src/amr/amr.cpp:484-564).
+__global__ void prolong_kernel(BlockDesc c, const double* __restrict__ cu, BlockDesc f, double* fu) {
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= f.stride) return;
+    const int I[3] = {(int)(t % f.nt[0]), (int)((t / f.nt[0]) % f.nt[1]), (int)(t / ((long)f.nt[0] * f.nt[1]))};
+    int c0[3], c1[3];
+    double ax[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double x = coord(f, d, I[d]);
+        const double cf = (x - coord(c, d, c.ng)) / c.dx[d] + (double)c.ng;
+        int ci = (int)round(cf);
+        ci = ci < 0 ? 0 : (ci > c.nt[d] - 1 ? c.nt[d] - 1 : ci);
+        const double fr = cf - (double)ci;
+        int cj = ci + (fr >= 0.0 ? 1 : -1);
+        cj = cj > c.nt[d] - 1 ? c.nt[d] - 1 : cj;
+        cj = cj < 0 ? 0 : cj;
+        c0[d] = ci;
+        c1[d] = cj;
+        ax[d] = fabs(fr);
+    }
+    const double w000 = (1 - ax[0]) * (1 - ax[1]) * (1 - ax[2]);
+    const double w100 = ax[0] * (1 - ax[1]) * (1 - ax[2]);
+    const double w010 = (1 - ax[0]) * ax[1] * (1 - ax[2]);
+    const double w001 = (1 - ax[0]) * (1 - ax[1]) * ax[2];
+    const double w110 = ax[0] * ax[1] * (1 - ax[2]);
+    const double w101 = ax[0] * (1 - ax[1]) * ax[2];
+    const double w011 = (1 - ax[0]) * ax[1] * ax[2];
+    const double w111 = ax[0] * ax[1] * ax[2];
+    auto at = [&](int a, int b2, int e) { return (long)a + (long)c.nt[0] * ((long)b2 + (long)c.nt[1] * e); };
+    const long p000 = at(c0[0], c0[1], c0[2]), p100 = at(c1[0], c0[1], c0[2]), p010 = at(c0[0], c1[1], c0[2]),
+               p001 = at(c0[0], c0[1], c1[2]), p110 = at(c1[0], c1[1], c0[2]), p101 = at(c1[0], c0[1], c1[2]),
+               p011 = at(c0[0], c1[1], c1[2]), p111 = at(c1[0], c1[1], c1[2]);
+    for (int v = 0; v < NV; ++v) {
+        const double* U = cu + v * c.stride;
+        fu[v * f.stride + t] = w000 * U[p000] + w100 * U[p100] + w010 * U[p010] + w001 * U[p001] + w110 * U[p110] +
+            w101 * U[p101] + w011 * U[p011] + w111 * U[p111];
+    }
+}
+
+// 2x2x2 average of fine cells into the covered coarse interior cells
+// (src/amr/amr.cpp:566-626, refinement ratio 2).
+__global__ void restrict_kernel(BlockDesc f, const double* __restrict__ fu, BlockDesc c, double* cu) {
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ncell = (long)c.n[0] * c.n[1] * c.n[2];
+    if (t >= ncell) return;
+    const int C[3] = {c.ng + (int)(t % c.n[0]), c.ng + (int)((t / c.n[0]) % c.n[1]),
+                      c.ng + (int)(t / ((long)c.n[0] * c.n[1]))};
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        b[d] = (int)round((coord(c, d, C[d]) - coord(f, d, 0)) / f.dx[d]) - 1;
+        if (!(b[d] >= f.ng && b[d] + 2 <= f.ng + f.n[d] + f.ng)) return;
+    }
+    const long dst = C[0] + (long)c.nt[0] * (C[1] + (long)c.nt[1] * C[2]);
+    for (int v = 0; v < NV; ++v) {
+        const double* F = fu + v * f.stride;
+        double s = 0.0;
+        for (int rk = 0; rk < 2; ++rk)
+            for (int rj = 0; rj < 2; ++rj)
+                for (int ri = 0; ri < 2; ++ri) s += F[(b[0] + ri) + (long)f.nt[0] * ((b[1] + rj) + (long)f.nt[1] * (b[2] + rk))];
+        cu[v * c.stride + dst] = s / 8;
+    }
 }
 
 // every ghost cell of buffer `sel` of every block of a table; ofs[i] = first ghost cell of
@@ -925,7 +1003,10 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     return sh[0];
 }
 
-__global__ __launch_bounds__(RED_T) void sumsq_kernel(BlockDesc b, const double* ham, double* partial, long per) {
+// sum over the interior cells of the squares of an array of ncomp values per cell
+// (cell-major, as the reference's ham and mom arrays)
+__global__ __launch_bounds__(RED_T) void sumsq_kernel(BlockDesc b, const double* ham, int ncomp, double* partial,
+                                                      long per) {
     __shared__ double sh[RED_T];
     const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
     const long beg = (long)blockIdx.x * per;
@@ -935,8 +1016,11 @@ __global__ __launch_bounds__(RED_T) void sumsq_kernel(BlockDesc b, const double*
         const int i = b.ng + (int)(t % b.n[0]);
         const int j = b.ng + (int)((t / b.n[0]) % b.n[1]);
         const int k = b.ng + (int)(t / ((long)b.n[0] * b.n[1]));
-        const double h = ham[i + (long)b.nt[0] * (j + (long)b.nt[1] * k)];
-        acc += h * h;
+        const long c = i + (long)b.nt[0] * (j + (long)b.nt[1] * k);
+        for (int a = 0; a < ncomp; ++a) {
+            const double h = ham[ncomp * c + a];
+            acc += h * h;
+        }
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -1120,17 +1204,28 @@ hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& s
 }
 
 hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream_t s) {
-    if (!B.table) return launch_fill_ghosts(B.one.b, B.one.buf[sel], mode, s);
+    if (!B.table) return launch_fill_ghosts(B.one.b, B.one.buf[sel], mode, s, 0x3f);
     fill_ghosts_batch_kernel<<<grid_for(B.nghost, 256), 256, 0, s>>>(B.table, B.ghost_ofs, B.nblocks, sel, mode);
     return hipGetLastError();
 }
 
-hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s) {
+hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s, int faces) {
     const long nA = 2L * b.ng * b.nt[1] * b.nt[0];
     const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
     const long nC = (long)b.n[2] * b.n[1] * 2 * b.ng;
     const long n = nA + nB + nC;
-    fill_ghosts_kernel<<<grid_for(n, 256), 256, 0, s>>>(b, u, mode);
+    fill_ghosts_kernel<<<grid_for(n, 256), 256, 0, s>>>(b, u, mode, faces);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu, hipStream_t s) {
+    prolong_kernel<<<grid_for(f.stride, 256), 256, 0, s>>>(c, cu, f, fu);
+    return hipGetLastError();
+}
+
+hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc& c, double* cu, hipStream_t s) {
+    const long n = (long)c.n[0] * c.n[1] * c.n[2];
+    restrict_kernel<<<grid_for(n, 256), 256, 0, s>>>(f, fu, c, cu);
     return hipGetLastError();
 }
 
@@ -1167,10 +1262,10 @@ static long reduce_split(const BlockDesc& b, int cap, int* nparts) {
     return per;
 }
 
-hipError_t launch_sumsq_interior(const BlockDesc& b, const double* ham, double* partial, int cap, int* nparts,
-                                 hipStream_t s) {
+hipError_t launch_sumsq_interior(const BlockDesc& b, const double* ham, int ncomp, double* partial, int cap,
+                                 int* nparts, hipStream_t s) {
     const long per = reduce_split(b, cap, nparts);
-    sumsq_kernel<<<*nparts, RED_T, 0, s>>>(b, ham, partial, per);
+    sumsq_kernel<<<*nparts, RED_T, 0, s>>>(b, ham, ncomp, partial, per);
     return hipGetLastError();
 }
 

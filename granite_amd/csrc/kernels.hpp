@@ -88,8 +88,14 @@ int stage_tiles(const BlockDesc& b);
 // ghost fill of buffer `sel` of every block of a batch (mode as launch_fill_ghosts)
 hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream_t s);
 
-// Ghost fill of one block: mode 0 Sommerfeld (chained X,Y,Z), 1 periodic, 2 outflow copy.
-hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s);
+// Ghost fill of one block: mode 0 Sommerfeld (chained X,Y,Z), 1 periodic, 2 outflow copy
+// of the faces set in `faces` (bit 2d low side, 2d+1 high side of axis d).
+hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s, int faces = 0x3f);
+
+// trilinear coarse -> fine into every fine cell (ghosts included), and 2x2x2 average of
+// the fine block into the covered coarse interior cells
+hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu, hipStream_t s);
+hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc& c, double* cu, hipStream_t s);
 
 // rhs += Kreiss-Oliger(u) on the interior (reference accumulation order)
 hipError_t launch_dissipation(const BlockDesc& b, const double* u, double* rhs, double ko_sigma, hipStream_t s);
@@ -104,10 +110,11 @@ hipError_t launch_init_bl(const BlockDesc& b, double* u, int nbh, const double* 
 hipError_t launch_add_noise(const BlockDesc& b, double* u, unsigned long long seed, double amp, unsigned mask,
                             hipStream_t s);
 
-// partial[blockIdx] = sum over interior of ham^2 ; count is analytic. Returns the number of
-// partials written in *nparts (<= cap).
-hipError_t launch_sumsq_interior(const BlockDesc& b, const double* ham, double* partial, int cap, int* nparts,
-                                 hipStream_t s);
+// partial[blockIdx] = sum over interior cells of the squares of the ncomp values per cell
+// (ncomp 1: ham, 3: mom); count is analytic. Returns the number of partials written in
+// *nparts (<= cap).
+hipError_t launch_sumsq_interior(const BlockDesc& b, const double* ham, int ncomp, double* partial, int cap,
+                                 int* nparts, hipStream_t s);
 // sums `n` doubles into out[0] with one workgroup (fixed order)
 hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
 // partial[blockIdx] = number of non-finite interior values over all 22 fields

@@ -165,6 +165,48 @@ int gr_level_constraints_l2(gr_level* lvl, double* sumsq, long long* count);
  * src/main.cpp:1192-1214). */
 int gr_level_nonfinite(gr_level* lvl, long long* count);
 
+/* ---- Berger-Oliger hierarchy of device levels ---------------------------------------------
+ *
+ * A hierarchy is an ordered list of levels (coarsest first, refinement ratio 2, every level's
+ * blocks created with gr_level_create on one engine), playing the part of
+ * amr::AMRHierarchy (include/granite/amr/amr.hpp:49-117) for a fixed geometry: regridding
+ * stays with the caller, who re-creates the levels and the hierarchy after it. The hierarchy
+ * borrows the levels; destroy it before them. Level 0 is the domain block(s). */
+typedef struct gr_hierarchy gr_hierarchy;
+
+/* Fails with GR_ERR_GEOM unless dx of level l+1 is dx of level l / 2 (relative 1e-6, the
+ * reference's regrid check at src/amr/amr.cpp:280-287) and GR_ERR_ARG unless all levels
+ * share the engine. */
+int gr_hierarchy_create(gr_engine* eng, int nlevels, gr_level* const* levels, gr_hierarchy** out);
+int gr_hierarchy_destroy(gr_hierarchy* h);
+
+/* AMRHierarchy::prolongate of every block of level-1 into every cell (ghosts included) of
+ * every block of `level` >= 1, in the reference's block order (src/amr/amr.cpp:484-564);
+ * what regrid does to a new level (src/amr/amr.cpp:290-293). */
+int gr_hierarchy_prolongate(gr_hierarchy* h, int level);
+/* AMRHierarchy::restrict_data of every block of `level` >= 1 into level-1
+ * (src/amr/amr.cpp:566-626). */
+int gr_hierarchy_restrict(gr_hierarchy* h, int level);
+/* AMRHierarchy::fillGhostZones(level) (src/amr/amr.cpp:400-482): level 0 outflow copy on
+ * all faces; level >= 1 prolongation from the parent level into all cells, then the outflow
+ * copy on the faces no parent block covers. */
+int gr_hierarchy_fill_ghosts(gr_hierarchy* h, int level);
+
+/* One coarse step: AMRHierarchy::subcycle(0, evolve) (src/amr/amr.cpp:303-350) with dt0 on
+ * level 0 and dt0 / 2^l on level l (setLevelDt, src/amr/amr.cpp:389-398), where evolve is
+ * the per-level SSP-RK3 step with 1/r ghosts after every stage (gr_level_rk3_step with
+ * GR_BC_SOMMERFELD and no pre-fill). Reproduces the reference's order exactly, including
+ * the re-prolongation of all fine cells after each fine step; no regrid. use_graph != 0
+ * replays the whole coarse step from one HIP graph captured on first use for the given
+ * (dt0, parameters, precision). */
+int gr_hierarchy_step(gr_hierarchy* h, const gr_ccz4_params* prm, double dt0, int precision, int use_graph);
+
+/* Constraint telemetry over the interiors of all blocks of all levels
+ * (src/main.cpp:1216-1250): sum of H^2, sum of |M|^2 (M_i the momentum constraint of
+ * computeConstraints) and the number of cells; ||H||_2 = sqrt(ham_sumsq / count). Either
+ * sum pointer may be null. */
+int gr_hierarchy_constraints_l2(gr_hierarchy* h, double* ham_sumsq, double* mom_sumsq, long long* count);
+
 /* ---- measurement (no reference counterpart) ----------------------------------------------- */
 
 /* enable != 0: every RK3 step run without a graph records HIP events around each of its

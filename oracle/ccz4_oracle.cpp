@@ -732,6 +732,50 @@ void orc_fill_ghosts(const int* n, int ng, const double* lo, const double* dx, d
     fill_bc(make_geom(n, ng, lo, dx), u, mode);
 }
 
+// The zero-gradient copy of AMRHierarchy::fillGhostZones (src/amr/amr.cpp:446-480) on the
+// faces whose bit is set in `faces` (bit 2d: low side of axis d, 2d+1: high side), in the
+// reference's order: x-low, x-high, y-low, y-high, z-low, z-high, each over the full extent of
+// the other two axes, ghosts included.
+// Pinned by: all six faces equal the mode-2 fill above; no face leaves the block unchanged;
+// one face changes only its own slab, to copies of the adjacent interior layer.
+void orc_fill_outflow_faces(const int* n, int ng, const double* lo, const double* dx, double* u, int nvars,
+                            int faces) {
+    Geom g = make_geom(n, ng, lo, dx);
+    auto at = [&](int v, int i, int j, int k) -> double& { return u[v * g.stride + g.at(i, j, k)]; };
+    for (int v = 0; v < nvars; ++v) {
+        if (faces & 1)
+            for (int k = 0; k < g.nt[2]; ++k)
+                for (int j = 0; j < g.nt[1]; ++j)
+                    for (int q = 0; q < ng; ++q) at(v, q, j, k) = at(v, ng, j, k);
+        if (faces & 2) {
+            const int ie = ng + g.n[0];
+            for (int k = 0; k < g.nt[2]; ++k)
+                for (int j = 0; j < g.nt[1]; ++j)
+                    for (int q = 0; q < ng; ++q) at(v, ie + q, j, k) = at(v, ie - 1, j, k);
+        }
+        if (faces & 4)
+            for (int k = 0; k < g.nt[2]; ++k)
+                for (int i = 0; i < g.nt[0]; ++i)
+                    for (int q = 0; q < ng; ++q) at(v, i, q, k) = at(v, i, ng, k);
+        if (faces & 8) {
+            const int je = ng + g.n[1];
+            for (int k = 0; k < g.nt[2]; ++k)
+                for (int i = 0; i < g.nt[0]; ++i)
+                    for (int q = 0; q < ng; ++q) at(v, i, je + q, k) = at(v, i, je - 1, k);
+        }
+        if (faces & 16)
+            for (int j = 0; j < g.nt[1]; ++j)
+                for (int i = 0; i < g.nt[0]; ++i)
+                    for (int q = 0; q < ng; ++q) at(v, i, j, q) = at(v, i, j, ng);
+        if (faces & 32) {
+            const int ke = ng + g.n[2];
+            for (int j = 0; j < g.nt[1]; ++j)
+                for (int i = 0; i < g.nt[0]; ++i)
+                    for (int q = 0; q < ng; ++q) at(v, i, j, ke + q) = at(v, i, j, ke - 1);
+        }
+    }
+}
+
 // d = a1*s1 + a2*s2 + a3*dt*r on interior cells
 void orc_combine(const int* n, int ng, const double* lo, const double* dx, double* dst, double a1, const double* s1,
                  double a2, const double* s2, double a3, double dt, const double* r) {

@@ -141,6 +141,10 @@ class Oracle:
         self.lib.orc_fill_ghosts(*g.cargs(), _p(u), C.c_int(mode))
         return u
 
+    def fill_outflow_faces(self, g: Geom, u, faces: int, nvars=NV):
+        self.lib.orc_fill_outflow_faces(*g.cargs(), _p(u), C.c_int(nvars), C.c_int(faces))
+        return u
+
     def combine(self, g: Geom, dst, a1, s1, a2, s2, a3, dt, r):
         self.lib.orc_combine(*g.cargs(), _p(dst), C.c_double(a1), _p(s1), C.c_double(a2), _p(s2), C.c_double(a3),
                              C.c_double(dt), _p(r))

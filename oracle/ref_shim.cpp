@@ -150,4 +150,74 @@ void ref_restrict(const int* fn, int fng, const double* flo, const double* fhi, 
     std::memcpy(cu, c->flatBuffer(), total(*c) * sizeof(double));
 }
 
+// ---- the reference's AMRHierarchy, driven as src/main.cpp drives it ---------------------
+
+struct ref_hier {
+    std::unique_ptr<amr::AMRHierarchy> h;
+};
+
+// AMRHierarchy over ncells on [dlo, dhi] with ng ghosts, max_levels levels and ratio 2,
+// regridding never during subcycle (regrid_interval 0); one tracking sphere per center with
+// the given radius and min_level, then initialize() with a tagger that tags nothing, so the
+// levels come from the tracking spheres alone (src/amr/amr.cpp:34-78, 80-301).
+ref_hier* ref_hier_build(const int* ncells, const double* dlo, const double* dhi, int ng, int max_levels,
+                         int nspheres, const double* centers, const double* radii, const int* min_levels) {
+    amr::AMRParams ap;
+    ap.max_levels = max_levels;
+    ap.refinement_ratio = 2;
+    ap.regrid_interval = 0;
+    SimulationParams sp;
+    for (int d = 0; d < 3; ++d) {
+        sp.ncells[d] = ncells[d];
+        sp.domain_lo[d] = dlo[d];
+        sp.domain_hi[d] = dhi[d];
+    }
+    sp.ghost_cells = ng;
+    auto* r = new ref_hier;
+    r->h = std::make_unique<amr::AMRHierarchy>(ap, sp);
+    for (int s = 0; s < nspheres; ++s)
+        r->h->addTrackingSphere({centers[3 * s], centers[3 * s + 1], centers[3 * s + 2]}, radii[s], min_levels[s]);
+    r->h->initialize([](const GridBlock&, int, int, int) { return false; });
+    return r;
+}
+
+void ref_hier_free(ref_hier* r) { delete r; }
+
+int ref_hier_nblocks(ref_hier* r) { return (int)r->h->getAllBlocks().size(); }
+
+// geometry of block i of getAllBlocks() (coarsest level first)
+void ref_hier_block(ref_hier* r, int i, int* level, int* n, double* lo, double* hi, double* dx) {
+    GridBlock* b = r->h->getAllBlocks()[i];
+    *level = b->getLevel();
+    for (int d = 0; d < 3; ++d) {
+        n[d] = b->numCells()[d];
+        lo[d] = b->lowerCorner()[d];
+        hi[d] = b->upperCorner()[d];
+        dx[d] = b->cellSpacing()[d];
+    }
+}
+
+double* ref_hier_data(ref_hier* r, int i) { return r->h->getAllBlocks()[i]->flatBuffer(); }
+
+void ref_hier_fill_ghosts(ref_hier* r, int level) { r->h->fillGhostZones(level); }
+
+// per-block evolve callback: the level's SSP-RK3 step on one block, in place
+typedef void (*ref_evolve_fn)(int level, const int* n, int ng, const double* lo, const double* dx, double* u,
+                              double dt);
+
+// one coarse step as src/main.cpp:1184-1187 takes it: setLevelDt(0, dt0), subcycle(0)
+void ref_hier_step(ref_hier* r, double dt0, ref_evolve_fn evolve) {
+    r->h->setLevelDt(0, dt0);
+    r->h->subcycle(
+        0,
+        [evolve](std::vector<GridBlock*>& blocks, Real dt) {
+            for (GridBlock* b : blocks) {
+                const int n[3] = {b->numCells()[0], b->numCells()[1], b->numCells()[2]};
+                evolve(b->getLevel(), n, b->getNumGhost(), b->lowerCorner().data(), b->cellSpacing().data(),
+                       b->flatBuffer(), dt);
+            }
+        },
+        [](const GridBlock&, int, int, int) { return false; });
+}
+
 } // extern "C"
