@@ -119,6 +119,53 @@ def run_batched(ga, eng, p, args):
     return out
 
 
+def run_hierarchy(ga, eng, p):
+    """The 4-level binary-puncture hierarchy (granite_amd.hierarchy.binary_puncture_levels:
+    96^3 on [-64,64]^3 plus tracking-sphere levels of 106x96^2, 114x96^2, 132x96^2 cells at
+    dx 2/3, 1/3, 1/6; punctures m = 0.5 at (+-3,0,0)) evolved to t = 10 M: 30 Berger-Oliger
+    coarse steps of dt0 = 1/3, each one replay of a captured HIP graph (level l steps 2^l
+    times). Value = interior cell updates of all levels per second, over coarse steps 2-30
+    (step 1 captures the graph). Constraint telemetry at t = 10 for FP64 and mixed."""
+    from granite_amd import hierarchy as hy
+
+    geo = hy.binary_puncture_levels()
+    per_step = hy.cell_steps_per_coarse_step(geo)
+    masses, pos = hy.PUNCTURES
+    dt0 = 0.25 * (128.0 / 96)
+    out = {"levels": [[n, lo, hi] for lv in geo for n, lo, hi in lv], "cell_updates_per_coarse_step": per_step,
+           "coarse_steps": 30, "dt0": dt0}
+    for name, prec in (("fp64", ga.PREC_FP64), ("mixed", ga.PREC_MIXED)):
+        levels = [ga.Level([ga.GridBlock(0, l, n, lo, hi, 4) for n, lo, hi in lv], eng) for l, lv in enumerate(geo)]
+        H = ga.Hierarchy(levels, eng)
+        levels[0].init_brill_lindquist(masses, pos)
+        for l in range(1, len(levels)):
+            H.prolongate(l)
+        H.fill_ghosts(0)
+        H.step(p, dt0, prec, use_graph=True)
+        H.sync()
+        t0 = time.perf_counter()
+        for _ in range(29):
+            H.step(p, dt0, prec, use_graph=True)
+        H.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / 29
+        sh, sm, n = H.constraints_l2()
+        bad = sum(lv.nonfinite() for lv in levels)
+        out[name] = {"ms_per_coarse_step": ms, "value": per_step / (ms * 1e-3), "ham_l2_t10": (sh / n) ** 0.5,
+                     "mom_l2_t10": (sm / n) ** 0.5, "nonfinite": bad}
+        H.close()
+        for lv in levels:
+            lv.close()
+    out["mixed_speedup_vs_fp64"] = out["fp64"]["ms_per_coarse_step"] / out["mixed"]["ms_per_coarse_step"]
+    out["mixed_ham_l2_rel_vs_fp64"] = abs(out["mixed"]["ham_l2_t10"] / out["fp64"]["ham_l2_t10"] - 1.0)
+    gpath = os.path.join(ROOT, "tests", "golden", "binary_hierarchy_t10.json")
+    if os.path.exists(gpath):
+        g = json.load(open(gpath))["telemetry"][-1]
+        ref = (g["ham_sumsq"] / g["count"]) ** 0.5
+        out["oracle_ham_l2_t10"] = ref
+        out["fp64_ham_l2_rel_vs_oracle"] = abs(out["fp64"]["ham_l2_t10"] / ref - 1.0)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +174,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-AMR-blocks section")
+    ap.add_argument("--no-hierarchy", action="store_true", help="skip the binary-puncture hierarchy section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -185,6 +233,7 @@ def main():
     f64 = run_form(ga.PREC_FP64)
     mix = run_form(ga.PREC_MIXED)
     batched = None if args.no_batched else run_batched(ga, eng, p, args)
+    hier = None if args.no_hierarchy else run_hierarchy(ga, eng, p)
     value = f64["value"]
     ms_per_step = f64["ms_per_step"]
 
@@ -251,6 +300,7 @@ def main():
         },
         "checks": {"nonfinite": f64["nonfinite"], "ham_l2": f64["ham_l2"], "t": f64["t"]},
         "batched_amr_blocks": batched,
+        "binary_hierarchy_t10": hier,
         "forms": {
             "fp64": f64,
             "mixed": dict(mix, speedup_vs_fp64=mix["value"] / value,

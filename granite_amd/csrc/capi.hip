@@ -90,7 +90,8 @@ int check_geom(gr_engine* e, const gr_geom* g, gr::BlockDesc* out) {
     if (g->ng < 4) return fail(e, GR_ERR_GEOM, "ng must be >= 4 (the upwind stencil reaches 4 cells)");
     const long nt0 = g->n[0] + 2L * g->ng, nt1 = g->n[1] + 2L * g->ng, nt2 = g->n[2] + 2L * g->ng;
     const long stride = nt0 * nt1 * nt2;
-    if (stride * 22 >= (1L << 31) * 8L) return fail(e, GR_ERR_GEOM, "block too large");
+    // the stage kernels address a field with 32-bit byte offsets
+    if (stride >= (1L << 29)) return fail(e, GR_ERR_GEOM, "block too large (over 2^29 cells with ghosts)");
     for (int d = 0; d < 3; ++d) {
         out->n[d] = g->n[d];
         out->nt[d] = g->n[d] + 2 * g->ng;

@@ -42,6 +42,21 @@
 
 namespace gr {
 
+// Device buffers reached through tables (block descriptors, stage selectors) are generic
+// pointers to the compiler, which would access them with flat instructions; flat accesses
+// also count against the LDS wait counter, so every LDS wait would wait for the in-flight
+// prefetch of the next field as well. The stage kernels access HBM through this type instead.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) char gchar;
+
+// element at byte offset `off` (< 4 GiB) from a workgroup-uniform base
+__device__ __forceinline__ double gload(const gdouble* base, unsigned off) {
+    return *(const gdouble*)((const gchar*)base + off);
+}
+__device__ __forceinline__ void gstore(gdouble* base, unsigned off, double v) {
+    *(gdouble*)((gchar*)base + off) = v;
+}
+
 namespace {
 
 constexpr int HALO = 4;
@@ -142,17 +157,18 @@ struct TileIO {
     static constexpr int LBUF = T::NLDS + 1; // one LDS buffer incl. a discard slot
 
     double* lds; // 2 * LBUF doubles
-    const double* in;
-    const double* un;
-    double* out;
+    const gdouble* in; // global address space: loads/stores wait on vmcnt only, never on LDS waits
+    const gdouble* un;
+    gdouble* out;
     long stride;
     long cidx;
+    unsigned cb; // byte offset of the thread's cell inside one field
     int cl;
     bool active;
     bool ko_on;
     StageCoef cf;
     int loff[NL]; // LDS slot of each owned entry (T::NLDS = discard slot)
-    int goff[NL]; // offset of the entry inside one field of the block
+    unsigned goff[NL]; // byte offset of the entry inside one field of the block
     double buf[PF][NL];
     double unb[PF];
     double unv[NV];
@@ -177,7 +193,7 @@ struct TileIO {
             const int gx = ox + x, gy = oy + y, gz = oz + z;
             const bool ok = p < T::TOTAL && gx < nt0 && gy < nt1 && gz < nt2;
             loff[i] = ok ? T::lds(x, y, z) : T::NLDS;
-            goff[i] = ok ? gx + nt0 * (gy + nt1 * gz) : 0;
+            goff[i] = ok ? 8u * (unsigned)(gx + nt0 * (gy + nt1 * gz)) : 0u;
         }
 #pragma unroll
         for (int k = 0; k < PF; ++k) prefetch(k);
@@ -189,10 +205,10 @@ struct TileIO {
         if (pos >= NV) return;
         const int v = FIELD_ORDER[pos];
         const int slot = pos % PF;
-        const double* f = in + v * stride;
+        const gdouble* f = in + v * stride;
 #pragma unroll
-        for (int i = 0; i < NL; ++i) buf[slot][i] = f[goff[i]];
-        if (MODE != MODE_RHS) unb[slot] = un[v * stride + cidx];
+        for (int i = 0; i < NL; ++i) buf[slot][i] = gload(f, goff[i]);
+        if (MODE != MODE_RHS) unb[slot] = gload(un + v * stride, cb);
     }
 
     // writes the prefetched field at position pos into its LDS buffer
@@ -206,10 +222,10 @@ struct TileIO {
     }
 
     // ---- Stager
-    __device__ double centre(int v) const { return in[v * stride + cidx]; }
+    __device__ double centre(int v) const { return gload(in + v * stride, cb); }
     __device__ void metric64(double g[6]) const {
 #pragma unroll
-        for (int a = 0; a < 6; ++a) g[a] = in[(GXX + a) * stride + cidx];
+        for (int a = 0; a < 6; ++a) g[a] = gload(in + (GXX + a) * stride, cb);
     }
     __device__ FixedStrides<1, T::BX, T::BX * T::BY> strides() const { return {}; }
     __device__ const double* field(int v) {
@@ -224,7 +240,7 @@ struct TileIO {
     __device__ __forceinline__ void put(int v, double r, double ko, double uin) {
         const double rr = ko_on ? r + ko : r;
         if (MODE == MODE_RHS) {
-            if (active) out[v * stride + cidx] = rr;
+            if (active) gstore(out + v * stride, cb, rr);
             return;
         }
         double val = combine_exact(cf.a1, unv[v], cf.a2, uin, cf.a3dt, rr);
@@ -238,7 +254,7 @@ struct TileIO {
             hA[v - AXX] = val;
             return;
         }
-        if (active) out[v * stride + cidx] = val;
+        if (active) gstore(out + v * stride, cb, val);
     }
 
     __device__ __forceinline__ void finish() {
@@ -247,8 +263,8 @@ struct TileIO {
         if (!active) return;
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
-            out[(GXX + a) * stride + cidx] = hg[a];
-            out[(AXX + a) * stride + cidx] = hA[a];
+            gstore(out + (GXX + a) * stride, cb, hg[a]);
+            gstore(out + (AXX + a) * stride, cb, hA[a]);
         }
     }
 };
@@ -267,18 +283,19 @@ struct TileIO32 {
     static constexpr int LBUF = T::NLDS + 1;
 
     float* lds; // 2 * LBUF floats
-    const double* in;
-    const double* un;
-    double* out;
+    const gdouble* in; // global address space: loads/stores wait on vmcnt only, never on LDS waits
+    const gdouble* un;
+    gdouble* out;
     long stride;
     long cidx;
+    unsigned cb; // byte offset of the thread's cell inside one field
     long ridx; // the tile's reference cell
     int cl;
     bool active;
     bool ko_on;
     StageCoef cf;
     int loff[NL];
-    int goff[NL];
+    unsigned goff[NL];
     double buf[PF][NL];
     double refb[PF], unb[PF], inb[PF];
     double unv[NV], inv[NV];
@@ -303,7 +320,7 @@ struct TileIO32 {
             const int gx = ox + x, gy = oy + y, gz = oz + z;
             const bool ok = p < T::TOTAL && gx < nt0 && gy < nt1 && gz < nt2;
             loff[i] = ok ? T::lds(x, y, z) : T::NLDS;
-            goff[i] = ok ? gx + nt0 * (gy + nt1 * gz) : 0;
+            goff[i] = ok ? 8u * (unsigned)(gx + nt0 * (gy + nt1 * gz)) : 0u;
         }
 #pragma unroll
         for (int k = 0; k < PF; ++k) prefetch(k);
@@ -315,12 +332,12 @@ struct TileIO32 {
         if (pos >= NV) return;
         const int v = FIELD_ORDER[pos];
         const int slot = pos % PF;
-        const double* f = in + v * stride;
+        const gdouble* f = in + v * stride;
 #pragma unroll
-        for (int i = 0; i < NL; ++i) buf[slot][i] = f[goff[i]];
+        for (int i = 0; i < NL; ++i) buf[slot][i] = gload(f, goff[i]);
         refb[slot] = f[ridx];
-        inb[slot] = f[cidx];
-        if (MODE != MODE_RHS) unb[slot] = un[v * stride + cidx];
+        inb[slot] = gload(f, cb);
+        if (MODE != MODE_RHS) unb[slot] = gload(un + v * stride, cb);
     }
 
     __device__ __forceinline__ void stage_to_lds(int pos) {
@@ -336,10 +353,10 @@ struct TileIO32 {
     }
 
     // ---- Stager
-    __device__ float centre(int v) const { return (float)in[v * stride + cidx]; }
+    __device__ float centre(int v) const { return (float)gload(in + v * stride, cb); }
     __device__ void metric64(double g[6]) const {
 #pragma unroll
-        for (int a = 0; a < 6; ++a) g[a] = in[(GXX + a) * stride + cidx];
+        for (int a = 0; a < 6; ++a) g[a] = gload(in + (GXX + a) * stride, cb);
     }
     __device__ FixedStrides<1, T::BX, T::BX * T::BY> strides() const { return {}; }
     __device__ const float* field(int v) {
@@ -354,7 +371,7 @@ struct TileIO32 {
     __device__ __forceinline__ void put(int v, float r, float ko, float) {
         const double rr = ko_on ? (double)r + (double)ko : (double)r;
         if (MODE == MODE_RHS) {
-            if (active) out[v * stride + cidx] = rr;
+            if (active) gstore(out + v * stride, cb, rr);
             return;
         }
         double val = combine_exact(cf.a1, unv[v], cf.a2, inv[v], cf.a3dt, rr);
@@ -368,7 +385,7 @@ struct TileIO32 {
             hA[v - AXX] = val;
             return;
         }
-        if (active) out[v * stride + cidx] = val;
+        if (active) gstore(out + v * stride, cb, val);
     }
 
     __device__ __forceinline__ void finish() {
@@ -377,8 +394,8 @@ struct TileIO32 {
         if (!active) return;
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
-            out[(GXX + a) * stride + cidx] = hg[a];
-            out[(AXX + a) * stride + cidx] = hA[a];
+            gstore(out + (GXX + a) * stride, cb, hg[a]);
+            gstore(out + (AXX + a) * stride, cb, hA[a]);
         }
     }
 };
@@ -408,12 +425,23 @@ __device__ __forceinline__ int tile_of_block(int bid, int ntiles, int remap) {
     return (bid & 7) * per + (bid >> 3);
 }
 
+// a value the whole workgroup holds alike, moved to scalar registers
+__device__ __forceinline__ unsigned long long wg_uniform(unsigned long long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+template <class P>
+__device__ __forceinline__ P* wg_uniform(P* p) {
+    return (P*)wg_uniform((unsigned long long)p);
+}
+
 // the block and local tile coordinates of global tile t (uniform per workgroup)
 struct TileRef {
     BlockDesc b;
-    const double* in;
-    const double* un;
-    double* out;
+    const gdouble* in; // global address space: loads/stores wait on vmcnt only, never on LDS waits
+    const gdouble* un;
+    gdouble* out;
     int tix, tiy, tiz;
 };
 
@@ -440,9 +468,11 @@ __device__ __forceinline__ TileRef resolve_tile(const StageArgs& A, int t) {
         tiles1 = A.batch.one.tiles[1];
     }
     const double* bufs[3] = {b0, b1, b2};
-    r.in = bufs[A.sel_in];
-    r.un = bufs[A.sel_un];
-    r.out = const_cast<double*>(bufs[A.sel_out]);
+    // t is the same for the whole workgroup, and so is everything read for it
+    r.in = wg_uniform((const gdouble*)bufs[A.sel_in]);
+    r.un = wg_uniform((const gdouble*)bufs[A.sel_un]);
+    r.out = wg_uniform((gdouble*)const_cast<double*>(bufs[A.sel_out]));
+    r.b.stride = (long)wg_uniform((unsigned long long)r.b.stride);
     const int lt = t - tile0;
     r.tix = lt % tiles0;
     r.tiy = (lt / tiles0) % tiles1;
@@ -473,6 +503,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(St
     io.out = R.out;
     io.stride = b.stride;
     io.cidx = cidx;
+    io.cb = 8u * (unsigned)cidx;
     io.cl = T::lds(lx, ly, lz);
     io.active = active;
     io.ko_on = A.ko_on != 0;
@@ -516,6 +547,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_m
     io.out = R.out;
     io.stride = b.stride;
     io.cidx = cidx;
+    io.cb = 8u * (unsigned)cidx;
     io.ridx = (b.ng + R.tix * TX) + (long)b.nt[0] * ((b.ng + R.tiy * TY) + (long)b.nt[1] * (b.ng + R.tiz * TZ));
     io.cl = T::lds(lx, ly, lz);
     io.active = active;
