@@ -39,6 +39,9 @@
 #ifndef GR_MIXED_PREFETCH
 #define GR_MIXED_PREFETCH 1
 #endif
+#ifndef GR_LDS_PAD32
+#define GR_LDS_PAD32 0
+#endif
 
 namespace gr {
 
@@ -278,7 +281,7 @@ struct TileIO {
 template <int TX, int TY, int TZ, int MODE, int PF>
 struct TileIO32 {
     // the FP32 tile measured faster without the row padding the FP64 tile uses
-    using T = TileGeom<TX, TY, TZ, false>;
+    using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0>;
     static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
     static constexpr int LBUF = T::NLDS + 1;
 
@@ -526,7 +529,7 @@ constexpr int FP64_TX = GR_FP64_TILE_X, FP64_TY = GR_FP64_TILE_Y, FP64_TZ = GR_F
 
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC>
 __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_mixed(StageArgs A) {
-    using T = TileGeom<TX, TY, TZ, false>;
+    using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0>;
     __shared__ float lds[2 * (T::NLDS + 1)];
 
     const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
