@@ -119,6 +119,51 @@ def run_batched(ga, eng, p, args):
     return out
 
 
+def run_noisy_192(ga, eng, p, steps):
+    """The mixed-precision single-puncture case: BL M = 1, one 192^3 block on [-16,16]^3
+    (dx = 1/6, dt = 1/24) with N(0, 1e-8^2) added to the conformal metric and K (seed
+    20240611, the engine's portable generator). Per-field max RHS error of the mixed form
+    relative to the FP64 form (tensor-group scale, all interior cells; the FP64 form matches
+    the oracle to 1e-12), then `steps` RK3 steps of each form with their ||H||_2."""
+    import numpy as np
+
+    n, half = 192, 16.0
+    dt = 0.25 * (2 * half / n)
+    noise_fields = [1, 2, 3, 4, 5, 6, 13]
+    out = {"block": [n] * 3, "dt": dt, "noise": 1e-8, "steps": steps}
+    groups = [[0], list(range(1, 7)), list(range(7, 13)), [13], [14, 15, 16], [17], [18], [19, 20, 21]]
+    res = {}
+    for name, prec in (("fp64", ga.PREC_FP64), ("mixed", ga.PREC_MIXED)):
+        blk = ga.GridBlock(0, 0, (n, n, n), (-half,) * 3, (half,) * 3, 4)
+        lvl = ga.Level([blk], eng)
+        lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
+        lvl.add_noise(20240611, 1e-8, noise_fields)
+        r = lvl.rhs(0, p, prec)[:, 4:-4, 4:-4, 4:-4]
+        res[name] = r
+        st = dict(precision=prec, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
+        lvl.rk3_step(p, dt, use_graph=True, **st)
+        lvl.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            lvl.rk3_step(p, dt, use_graph=True, **st)
+        lvl.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / max(steps - 1, 1)
+        sumsq, count = lvl.constraints_l2()
+        out[name] = {"ms_per_step": ms, "value": n ** 3 / (ms * 1e-3), "ham_l2": (sumsq / count) ** 0.5,
+                     "nonfinite": lvl.nonfinite(), "t": steps * dt}
+        lvl.close()
+    a, b = res["mixed"], res["fp64"]
+    scale = np.empty(22)
+    for grp in groups:
+        scale[grp] = max(float(np.max(np.abs(b[grp]))), 1e-300)
+    err = [float(np.max(np.abs(a[v] - b[v])) / scale[v]) for v in range(22)]
+    out["mixed_rhs_rel_err_per_field"] = err
+    out["mixed_rhs_rel_err_max"] = max(err)
+    out["mixed_speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    out["mixed_ham_l2_rel_vs_fp64"] = abs(out["mixed"]["ham_l2"] / out["fp64"]["ham_l2"] - 1.0)
+    return out
+
+
 def run_hierarchy(ga, eng, p):
     """The 4-level binary-puncture hierarchy (granite_amd.hierarchy.binary_puncture_levels:
     96^3 on [-64,64]^3 plus tracking-sphere levels of 106x96^2, 114x96^2, 132x96^2 cells at
@@ -175,6 +220,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-AMR-blocks section")
     ap.add_argument("--no-hierarchy", action="store_true", help="skip the binary-puncture hierarchy section")
+    ap.add_argument("--no-noisy", action="store_true", help="skip the 192^3 noisy mixed-precision section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -234,6 +280,7 @@ def main():
     mix = run_form(ga.PREC_MIXED)
     batched = None if args.no_batched else run_batched(ga, eng, p, args)
     hier = None if args.no_hierarchy else run_hierarchy(ga, eng, p)
+    noisy = None if args.no_noisy else run_noisy_192(ga, eng, p, 100)
     value = f64["value"]
     ms_per_step = f64["ms_per_step"]
 
@@ -301,6 +348,7 @@ def main():
         "checks": {"nonfinite": f64["nonfinite"], "ham_l2": f64["ham_l2"], "t": f64["t"]},
         "batched_amr_blocks": batched,
         "binary_hierarchy_t10": hier,
+        "noisy_192_mixed": noisy,
         "forms": {
             "fp64": f64,
             "mixed": dict(mix, speedup_vs_fp64=mix["value"] / value,

@@ -111,3 +111,42 @@ def test_mixed_evolution_matches_fp64_constraint(engine):
     say(f"H L2 fp64 {h64:.6e} mixed {h32:.6e} rel {abs(h32 - h64) / h64:.2e}")
     assert abs(h32 - h64) <= 0.01 * h64
     assert group_rel(u32, u64).max() < 1e-4
+
+
+def test_c3_full_size_rhs_error(engine, oracle):
+    """192^3 puncture block on [-16,16]^3 with N(0, 1e-8^2) on the conformal metric and K:
+    the mixed RHS against the FP64 form on every cell, and both against the oracle on three
+    sub-boxes (the RHS of a cell depends only on its stencil, so a sub-box with its own
+    ghosts cut from the full block reproduces it)."""
+    import granite_amd as ga
+
+    g = Geom((192, 192, 192), 4, (-16.0,) * 3, (16.0,) * 3)
+    blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
+    lvl = ga.Level([blk], engine)
+    lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
+    lvl.add_noise(20240611, 1e-8, [1, 2, 3, 4, 5, 6, 13])
+    lvl.download()
+    p = ga.CCZ4Params()
+    say("C3 192^3: mixed and FP64 RHS")
+    mixed = lvl.rhs(0, p, ga.PREC_MIXED)
+    fp64 = lvl.rhs(0, p, ga.PREC_FP64)
+    I = g.interior()
+    e = group_rel(mixed[I], fp64[I])
+    say("C3 mixed vs FP64 (all cells) " + np.array2string(e, precision=1))
+    assert e.max() < 1e-4
+    u = blk.array
+    ng = g.ng
+    for name, c0 in (("puncture", (80, 80, 80)), ("middle", (120, 96, 40)), ("far", (8, 150, 160))):
+        n = 32
+        sl = (slice(None),) + tuple(slice(c0[2 - a], c0[2 - a] + n + 2 * ng) for a in range(3))
+        lo = [g.lo[d] + c0[d] * g.dx[d] for d in range(3)]
+        sub = Geom((n, n, n), ng, lo, [lo[d] + n * g.dx[d] for d in range(3)])
+        su = np.ascontiguousarray(u[sl])
+        ref = oracle.rhs(sub, su)
+        J = (slice(None),) + tuple(slice(c0[2 - a] + ng, c0[2 - a] + ng + n) for a in range(3))
+        e64 = group_rel(fp64[J], ref[sub.interior()])
+        e32 = group_rel(mixed[J], ref[sub.interior()])
+        say(f"C3 sub-box {name}: fp64 {e64.max():.1e} mixed {e32.max():.1e}")
+        assert e64.max() < 1e-12
+        assert e32.max() < 1e-4
+    lvl.close()
