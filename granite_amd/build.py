@@ -28,6 +28,7 @@ FLAGS = [
     # packed-FP32 instructions stay disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    "-Wl,-soname,libgranite_ccz4.so",
 ]
 
 
@@ -45,5 +46,29 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     return out
 
 
+DRIVER_SRC = os.path.join(HERE, "host", "granite_main.cpp")
+DRIVER = os.path.join(HERE, "bin", "granite_main")
+
+
+def build_driver(verbose: bool = False, lib: str = LIB, out: str = DRIVER) -> str:
+    """The granite_main driver (host C++17 over the C ABI), linked to the in-tree library
+    (found at run time through $ORIGIN, so the tree can move)."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    tmp = out + ".tmp"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"),
+           DRIVER_SRC, "-L", os.path.dirname(lib), "-lgranite_ccz4",
+           "-Wl,-rpath,$ORIGIN/" + os.path.relpath(os.path.dirname(lib), os.path.dirname(out)),
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_driver(verbose="-v" in sys.argv))

@@ -113,6 +113,7 @@ def library():
         "gr_level_sync": (C.c_int, [vp]),
         "gr_level_constraints_l2": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
         "gr_level_nonfinite": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+        "gr_level_read_cell": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
         "gr_level_profile": (C.c_int, [vp, C.c_int]),
         "gr_level_set_mixed_policy": (C.c_int, [vp, C.c_double]),
         "gr_level_rhs": (C.c_int, [vp, C.c_int, C.POINTER(_Params), C.c_int, _dp]),
@@ -391,6 +392,12 @@ class Level:
         n = C.c_longlong()
         self._e.check(self._e.lib.gr_level_nonfinite(self._h, C.byref(n)))
         return n.value
+
+    def read_cell(self, i, var, x, y, z):
+        """Field var of block i at cell (x, y, z) counted with ghosts (GridBlock::data)."""
+        v = C.c_double()
+        self._e.check(self._e.lib.gr_level_read_cell(self._h, int(i), int(var), int(x), int(y), int(z), C.byref(v)))
+        return v.value
 
     def set_mixed_policy(self, chi_fp64_below):
         self._e.check(self._e.lib.gr_level_set_mixed_policy(self._h, float(chi_fp64_below)))

@@ -11,6 +11,11 @@
 
 #include <cmath>
 #include <cstdio>
+
+// run the FP64 tiles of the mixed form on a second stream, beside the FP32 kernel
+#ifndef GR_MIXED_OVERLAP
+#define GR_MIXED_OVERLAP 1
+#endif
 #include <cstring>
 #include <string>
 #include <vector>
@@ -18,6 +23,7 @@
 struct gr_engine {
     int device = -1;
     hipStream_t stream = nullptr;
+    gr::AuxStream aux{nullptr, nullptr, nullptr}; // runs the FP64 tiles beside the FP32 ones
     std::string err;
     // compatibility-mode staging buffers (grown on demand, doubles)
     double* d_u = nullptr;
@@ -157,6 +163,14 @@ int gr_engine_create(int device, gr_engine** out) {
         delete e;
         return GR_ERR_NODEV;
     }
+#if GR_MIXED_OVERLAP
+    if (hipStreamCreate(&e->aux.stream) != hipSuccess ||
+        hipEventCreateWithFlags(&e->aux.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->aux.join, hipEventDisableTiming) != hipSuccess) {
+        gr_engine_destroy(e);
+        return GR_ERR_NODEV;
+    }
+#endif
     *out = e;
     return GR_OK;
 }
@@ -169,6 +183,10 @@ int gr_engine_destroy(gr_engine* e) {
     if (e->d_r) (void)hipFree(e->d_r);
     if (e->d_src) (void)hipFree(e->d_src);
     if (e->d_aux) (void)hipFree(e->d_aux);
+    if (e->aux.stream) (void)hipStreamSynchronize(e->aux.stream);
+    if (e->aux.fork) (void)hipEventDestroy(e->aux.fork);
+    if (e->aux.join) (void)hipEventDestroy(e->aux.join);
+    if (e->aux.stream) (void)hipStreamDestroy(e->aux.stream);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return GR_OK;
@@ -496,7 +514,7 @@ static hipError_t stage(gr_level* L, const gr::Batch& B, const int sel[3], const
     const gr::CellParams P = cell_params(prm);
     if (precision == GR_PREC_MIXED)
         return gr::launch_stage_mixed(B, sel, none, P, prm->ko_sigma, mode, c, L->eng->stream, L->flags,
-                                      L->chi_fp64_below);
+                                      L->chi_fp64_below, L->eng->aux.stream ? &L->eng->aux : nullptr);
     return gr::launch_stage_fp64(B, sel, none, P, prm->ko_sigma, mode, c, L->eng->stream);
 }
 
@@ -644,6 +662,19 @@ int gr_level_constraints_l2(gr_level* L, double* sumsq, long long* count) {
     }
     *sumsq = total;
     *count = n;
+    return GR_OK;
+}
+
+int gr_level_read_cell(gr_level* L, int b, int var, int i, int j, int k, double* value) {
+    int rc = level_block(L, b);
+    if (rc) return rc;
+    gr_engine* e = L->eng;
+    const gr::BlockDesc& B = L->blk[b];
+    if (!value || var < 0 || var >= 22 || i < 0 || j < 0 || k < 0 || i >= B.nt[0] || j >= B.nt[1] || k >= B.nt[2])
+        return fail(e, GR_ERR_ARG, "cell or field index out of range");
+    const long off = var * B.stride + i + (long)B.nt[0] * (j + (long)B.nt[1] * k);
+    GR_HIP(e, hipMemcpyAsync(value, L->u[b] + off, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
     return GR_OK;
 }
 

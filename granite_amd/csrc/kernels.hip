@@ -417,6 +417,8 @@ struct StageArgs {
     // tiles whose flag differs from `want` are left to the other precision's kernel
     const unsigned char* flags;
     int want;
+    // scan != 0: launched as stage_kernel_scan over a small grid
+    int scan;
 };
 
 // workgroup id -> tile id: consecutive tiles go to the same XCD (workgroups are dealt
@@ -484,12 +486,8 @@ __device__ __forceinline__ TileRef resolve_tile(const StageArgs& A, int t) {
 }
 
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
-__global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
+__device__ __forceinline__ void stage_tile(const StageArgs& A, int tile, double* lds) {
     using T = TileGeom<TX, TY, TZ>;
-    __shared__ double lds[2 * (T::NLDS + 1)];
-
-    const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
-    if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
     const TileRef R = resolve_tile(A, tile);
     const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
@@ -525,7 +523,44 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(St
     ccz4_cell<double, HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
 }
 
+// one tile per workgroup
+template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
+__global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
+    using T = TileGeom<TX, TY, TZ>;
+    __shared__ double lds[2 * (T::NLDS + 1)];
+    const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
+    if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
+    stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, tile, lds);
+}
+
+// the flagged tiles only, from a small grid: each workgroup gathers the tiles with
+// flags == want among tiles blockIdx.x + k * gridDim.x and runs them one after the other
+// (every thread takes the same trip counts, so the barriers are uniform)
+template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
+__global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_scan(StageArgs A) {
+    using T = TileGeom<TX, TY, TZ>;
+    __shared__ double lds[2 * (T::NLDS + 1)];
+    __shared__ int list[T::NT];
+    __shared__ int nlist;
+    const int per = (int)gridDim.x * T::NT;
+    for (int base = 0; base < A.ntiles; base += per) {
+        if (threadIdx.x == 0) nlist = 0;
+        __syncthreads();
+        const int t = base + (int)blockIdx.x + (int)threadIdx.x * (int)gridDim.x;
+        if (t < A.ntiles && A.flags[t] == A.want) list[atomicAdd(&nlist, 1)] = t;
+        __syncthreads();
+        const int n = nlist;
+        __syncthreads();
+        for (int item = 0; item < n; ++item) {
+            stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, list[item], lds);
+            __syncthreads(); // the next tile reuses the LDS buffers
+        }
+    }
+}
+
 constexpr int FP64_TX = GR_FP64_TILE_X, FP64_TY = GR_FP64_TILE_Y, FP64_TZ = GR_FP64_TILE_Z;
+// workgroups of the FP64 pass of the mixed form: one per CU
+constexpr int FP64_SCAN_GRID = 256;
 
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC>
 __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_mixed(StageArgs A) {
@@ -593,8 +628,13 @@ __global__ __launch_bounds__(TX* TY* TZ) void classify_tiles_kernel(StageArgs A,
 
 template <int MODE, bool HAS_SRC>
 hipError_t launch_stage_t(const StageArgs& A, int nblk, hipStream_t s) {
-    stage_kernel<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0>
-        <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
+    if (nblk < 1) return hipSuccess;
+    if (A.scan)
+        stage_kernel_scan<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0>
+            <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
+    else
+        stage_kernel<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0>
+            <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
     return hipGetLastError();
 }
 
@@ -630,28 +670,34 @@ __device__ __forceinline__ long ghost_count(const BlockDesc& b) {
 // (bit 2d: low side of axis d, bit 2d+1: high side); the others keep their values, as
 // fillGhostZones leaves the faces covered by a parent level to prolongation
 // (src/amr/amr.cpp:412-480).
-__device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t, int mode, int faces = 0x3f) {
+__device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t_, int mode, int faces = 0x3f) {
 #pragma clang fp contract(off)
-    const long nA = 2L * b.ng * b.nt[1] * b.nt[0];
-    const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
+    // ghost cells number fewer than 2^29 (the block limit): 32-bit index arithmetic
+    const int t = (int)t_;
+    const int nt0 = b.nt[0], nt1 = b.nt[1];
+    const int nA = 2 * b.ng * nt1 * nt0;
+    const int nB = b.n[2] * 2 * b.ng * nt0;
     const int ng = b.ng, g2 = 2 * ng;
     int i, j, k;
     if (t < nA) {
-        i = (int)(t % b.nt[0]);
-        j = (int)((t / b.nt[0]) % b.nt[1]);
-        const int kk = (int)(t / ((long)b.nt[0] * b.nt[1]));
+        i = t % nt0;
+        const int r = t / nt0;
+        j = r % nt1;
+        const int kk = r / nt1;
         k = kk < ng ? kk : b.n[2] + kk;
     } else if (t < nA + nB) {
-        const long q = t - nA;
-        i = (int)(q % b.nt[0]);
-        const int jj = (int)((q / b.nt[0]) % g2);
-        k = ng + (int)(q / ((long)b.nt[0] * g2));
+        const int q = t - nA;
+        i = q % nt0;
+        const int r = q / nt0;
+        const int jj = r % g2;
+        k = ng + r / g2;
         j = jj < ng ? jj : b.n[1] + jj;
     } else {
-        const long q = t - nA - nB;
-        const int ii = (int)(q % g2);
-        j = ng + (int)((q / g2) % b.n[1]);
-        k = ng + (int)(q / ((long)g2 * b.n[1]));
+        const int q = t - nA - nB;
+        const int ii = q % g2;
+        const int r = q / g2;
+        j = ng + r % b.n[1];
+        k = ng + r / b.n[1];
         i = ii < ng ? ii : b.n[0] + ii;
     }
     const int I[3] = {i, j, k};
@@ -670,33 +716,31 @@ __device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t, int mode,
             A[d] = (lo && fill) ? ng : ((hi && fill) ? ng + b.n[d] - 1 : I[d]);
         }
     }
-    const long dst = I[0] + (long)b.nt[0] * (I[1] + (long)b.nt[1] * I[2]);
-    const long src = A[0] + (long)b.nt[0] * (A[1] + (long)b.nt[1] * A[2]);
-    if (mode != 0) {
-        for (int v = 0; v < NV; ++v) u[v * b.stride + dst] = u[v * b.stride + src];
-        return;
-    }
-    // (r_int, r_gh) of each step of the chain: start at the anchor, move one ghost axis
-    // at a time in X, Y, Z order
-    double rin[3], rgh[3];
-    int nsteps = 0;
-    {
+    const long dst = I[0] + (long)nt0 * (I[1] + (long)nt1 * I[2]);
+    const long src = A[0] + (long)nt0 * (A[1] + (long)nt1 * A[2]);
+    // the anchor is never a cell this launch writes, so all 22 loads go out before any store
+    double val[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) val[v] = u[v * b.stride + src];
+    if (mode == 0) {
+        // (r_int, r_gh) of each step of the chain: start at the anchor, move one ghost axis
+        // at a time in X, Y, Z order
         double Q[3] = {coord(b, 0, A[0]), coord(b, 1, A[1]), coord(b, 2, A[2])};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (!gh[d]) continue;
-            rin[nsteps] = radius(Q[0], Q[1], Q[2]);
+            const double rin = radius(Q[0], Q[1], Q[2]);
             Q[d] = coord(b, d, I[d]);
-            rgh[nsteps] = radius(Q[0], Q[1], Q[2]);
-            ++nsteps;
+            const double rgh = radius(Q[0], Q[1], Q[2]);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double fi = f_inf(v);
+                val[v] = fi + (val[v] - fi) * rin / rgh;
+            }
         }
     }
-    for (int v = 0; v < NV; ++v) {
-        const double fi = f_inf(v);
-        double val = u[v * b.stride + src];
-        for (int s = 0; s < nsteps; ++s) val = fi + (val - fi) * rin[s] / rgh[s];
-        u[v * b.stride + dst] = val;
-    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) u[v * b.stride + dst] = val[v];
 }
 
 __global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode, int faces) {
@@ -711,11 +755,11 @@ __global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode, int faces) 
 
 // Trilinear coarse -> fine into every fine cell, ghosts included, with the reference's
 // index rounding and clamping (src/amr/amr.cpp:484-564).
-__global__ void prolong_kernel(BlockDesc c, const double* __restrict__ cu, BlockDesc f, double* fu) {
+__global__ void prolong_kernel(BlockDesc c, const double* __restrict__ cu, BlockDesc f, double* __restrict__ fu) {
 #pragma clang fp contract(off)
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); // stride < 2^29
     if (t >= f.stride) return;
-    const int I[3] = {(int)(t % f.nt[0]), (int)((t / f.nt[0]) % f.nt[1]), (int)(t / ((long)f.nt[0] * f.nt[1]))};
+    const int I[3] = {t % f.nt[0], (t / f.nt[0]) % f.nt[1], t / (f.nt[0] * f.nt[1])};
     int c0[3], c1[3];
     double ax[3];
 #pragma unroll
@@ -744,6 +788,7 @@ __global__ void prolong_kernel(BlockDesc c, const double* __restrict__ cu, Block
     const long p000 = at(c0[0], c0[1], c0[2]), p100 = at(c1[0], c0[1], c0[2]), p010 = at(c0[0], c1[1], c0[2]),
                p001 = at(c0[0], c0[1], c1[2]), p110 = at(c1[0], c1[1], c0[2]), p101 = at(c1[0], c0[1], c1[2]),
                p011 = at(c0[0], c1[1], c1[2]), p111 = at(c1[0], c1[1], c1[2]);
+#pragma unroll 4
     for (int v = 0; v < NV; ++v) {
         const double* U = cu + v * c.stride;
         fu[v * f.stride + t] = w000 * U[p000] + w100 * U[p100] + w010 * U[p010] + w001 * U[p001] + w110 * U[p110] +
@@ -753,13 +798,12 @@ __global__ void prolong_kernel(BlockDesc c, const double* __restrict__ cu, Block
 
 // 2x2x2 average of fine cells into the covered coarse interior cells
 // (src/amr/amr.cpp:566-626, refinement ratio 2).
-__global__ void restrict_kernel(BlockDesc f, const double* __restrict__ fu, BlockDesc c, double* cu) {
+__global__ void restrict_kernel(BlockDesc f, const double* __restrict__ fu, BlockDesc c, double* __restrict__ cu) {
 #pragma clang fp contract(off)
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ncell = (long)c.n[0] * c.n[1] * c.n[2];
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); // interior cells < 2^29
+    const int ncell = c.n[0] * c.n[1] * c.n[2];
     if (t >= ncell) return;
-    const int C[3] = {c.ng + (int)(t % c.n[0]), c.ng + (int)((t / c.n[0]) % c.n[1]),
-                      c.ng + (int)(t / ((long)c.n[0] * c.n[1]))};
+    const int C[3] = {c.ng + t % c.n[0], c.ng + (t / c.n[0]) % c.n[1], c.ng + t / (c.n[0] * c.n[1])};
     int b[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -767,6 +811,7 @@ __global__ void restrict_kernel(BlockDesc f, const double* __restrict__ fu, Bloc
         if (!(b[d] >= f.ng && b[d] + 2 <= f.ng + f.n[d] + f.ng)) return;
     }
     const long dst = C[0] + (long)c.nt[0] * (C[1] + (long)c.nt[1] * C[2]);
+#pragma unroll 4
     for (int v = 0; v < NV; ++v) {
         const double* F = fu + v * f.stride;
         double s = 0.0;
@@ -1140,6 +1185,7 @@ static StageArgs stage_args(const Batch& B, const int sel[3], const Sources& src
     A.ko_on = ko_sigma > 0.0 ? 1 : 0;
     A.flags = flags;
     A.want = want;
+    A.scan = 0;
     return A;
 }
 
@@ -1195,16 +1241,23 @@ Batch single_batch(const BlockDesc& b, const double* in, const double* un, doubl
 hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags) {
-    const StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 1);
+    StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 1);
+    // with flags (the FP64 tiles of the mixed form: few, near the puncture) a small grid
+    // gathers them instead of launching one workgroup per tile of the level
+    int nblk = A.ntiles;
+    if (flags) {
+        A.scan = 1;
+        nblk = A.ntiles < FP64_SCAN_GRID ? A.ntiles : FP64_SCAN_GRID;
+    }
     const bool has_src = src.rho != nullptr;
     if (has_src) {
-        if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, true>(A, A.ntiles, s);
-        if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, true>(A, A.ntiles, s);
-        return launch_stage_t<MODE_FINAL, true>(A, A.ntiles, s);
+        if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, true>(A, nblk, s);
+        if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, true>(A, nblk, s);
+        return launch_stage_t<MODE_FINAL, true>(A, nblk, s);
     }
-    if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, false>(A, A.ntiles, s);
-    if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, false>(A, A.ntiles, s);
-    return launch_stage_t<MODE_FINAL, false>(A, A.ntiles, s);
+    if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, false>(A, nblk, s);
+    if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, false>(A, nblk, s);
+    return launch_stage_t<MODE_FINAL, false>(A, nblk, s);
 }
 
 template <int MODE, bool HAS_SRC>
@@ -1216,12 +1269,20 @@ static hipError_t launch_mixed_t(const StageArgs& A, hipStream_t s) {
 
 hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
                               double ko_sigma, int mode, const StageCoef& cf, hipStream_t s, unsigned char* flags,
-                              double chi_fp64_below) {
+                              double chi_fp64_below, const AuxStream* aux) {
     const StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 0);
     classify_tiles_kernel<FP64_TX, FP64_TY, FP64_TZ>
         <<<dim3(A.ntiles), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A, chi_fp64_below, flags);
     hipError_t st = hipGetLastError();
     if (st != hipSuccess) return st;
+    // the FP64 tiles (flag 1) run beside the FP32 kernel on the auxiliary stream: both read
+    // the stage input and write disjoint tiles of the output
+    if (aux) {
+        if ((st = hipEventRecord(aux->fork, s)) != hipSuccess) return st;
+        if ((st = hipStreamWaitEvent(aux->stream, aux->fork, 0)) != hipSuccess) return st;
+        if ((st = launch_stage_fp64(B, sel, src, P, ko_sigma, mode, cf, aux->stream, flags)) != hipSuccess) return st;
+        if ((st = hipEventRecord(aux->join, aux->stream)) != hipSuccess) return st;
+    }
     // FP32 tiles (flag 0)
     const bool has_src = src.rho != nullptr;
     if (has_src) {
@@ -1234,6 +1295,7 @@ hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& s
         else st = launch_mixed_t<MODE_FINAL, false>(A, s);
     }
     if (st != hipSuccess) return st;
+    if (aux) return hipStreamWaitEvent(s, aux->join, 0);
     // FP64 tiles near the puncture (flag 1)
     return launch_stage_fp64(B, sel, src, P, ko_sigma, mode, cf, s, flags);
 }

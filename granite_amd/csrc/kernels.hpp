@@ -77,12 +77,20 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& sr
                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags = nullptr);
 
+// A second stream (default priority) with the two events that fork work onto it from the
+// main stream and join it back.
+struct AuxStream {
+    hipStream_t stream;
+    hipEvent_t fork, join;
+};
+
 // Mixed-precision stage: classifies the tiles of the input (FP64 where the smallest chi of
 // the tile is below chi_fp64_below), then runs the FP32 kernel on the other tiles and the
-// FP64 kernel on those. flags: device buffer of B.ntiles bytes.
+// FP64 kernel on those (on `aux` beside the FP32 kernel when given, else after it on s).
+// flags: device buffer of B.ntiles bytes.
 hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
                               double ko_sigma, int mode, const StageCoef& cf, hipStream_t s, unsigned char* flags,
-                              double chi_fp64_below);
+                              double chi_fp64_below, const AuxStream* aux = nullptr);
 // number of stage tiles of a block
 int stage_tiles(const BlockDesc& b);
 // ghost fill of buffer `sel` of every block of a batch (mode as launch_fill_ghosts)

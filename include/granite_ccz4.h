@@ -165,6 +165,11 @@ int gr_level_constraints_l2(gr_level* lvl, double* sumsq, long long* count);
  * src/main.cpp:1192-1214). */
 int gr_level_nonfinite(gr_level* lvl, long long* count);
 
+/* One value of a block's device state: field var at cell (i, j, k) counted with ghosts, as
+ * GridBlock::data(var, i, j, k) (include/granite/core/grid.hpp:107-114), e.g. the lapse at
+ * the grid centre that the telemetry line prints (src/main.cpp:1238-1244). Synchronous. */
+int gr_level_read_cell(gr_level* lvl, int block, int var, int i, int j, int k, double* value);
+
 /* ---- Berger-Oliger hierarchy of device levels ---------------------------------------------
  *
  * A hierarchy is an ordered list of levels (coarsest first, refinement ratio 2, every level's
