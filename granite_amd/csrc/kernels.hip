@@ -16,10 +16,10 @@
 #define GR_FP64_TILE_X 8
 #endif
 #ifndef GR_FP64_TILE_Y
-#define GR_FP64_TILE_Y 8
+#define GR_FP64_TILE_Y 4
 #endif
 #ifndef GR_FP64_TILE_Z
-#define GR_FP64_TILE_Z 4
+#define GR_FP64_TILE_Z 8
 #endif
 #ifndef GR_FP64_PREFETCH
 #define GR_FP64_PREFETCH 1
@@ -40,7 +40,7 @@
 #define GR_MIXED_PREFETCH 1
 #endif
 #ifndef GR_LDS_PAD32
-#define GR_LDS_PAD32 0
+#define GR_LDS_PAD32 1
 #endif
 
 namespace gr {
@@ -280,7 +280,6 @@ struct TileIO {
 // an O(1) field (1e-8 noise on the metric) keeps FP32 relative precision of the delta.
 template <int TX, int TY, int TZ, int MODE, int PF>
 struct TileIO32 {
-    // the FP32 tile measured faster without the row padding the FP64 tile uses
     using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0>;
     static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
     static constexpr int LBUF = T::NLDS + 1;

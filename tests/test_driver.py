@@ -1,0 +1,67 @@
+"""granite_main (granite_amd/host/granite_main.cpp) on the CPU: parameter parsing and the
+level geometry it prints (--dry-run touches no GPU) against granite_amd.hierarchy, which is
+pinned to the reference's regrid. The GPU run is in test_gpu_driver.py."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from granite_amd import build as B
+from granite_amd.hierarchy import puncture_spheres, tracking_levels
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "driver")
+OFFSET = 1.3621415e-6  # the reference driver's grid phase shift (src/main.cpp:835-841)
+LEVEL = re.compile(r"AMR: Level (\d+)\s+block_id=(\d+)\s+ncells=\((\d+)x(\d+)x(\d+)\)\s+lo=\(([^)]*)\)\s+hi=\(([^)]*)\)")
+
+
+def run(*args, check=False):
+    if not os.path.exists(B.DRIVER):
+        pytest.skip("driver not built")
+    res = subprocess.run([B.DRIVER, *args], capture_output=True, text=True, timeout=60)
+    if check:
+        assert res.returncode == 0, res.stderr
+    return res
+
+
+def test_dry_run_geometry_matches_tracking_levels():
+    out = run(os.path.join(FIX, "bl_binary_small.yaml"), "--dry-run", check=True).stdout
+    assert "Grid: 24 x 24 x 24" in out and "dx = 1, dt = 0.25" in out and "t_final = 1.5" in out
+    assert "Brill-Lindquist: 2 BH(s), ADM mass = 1 M_sun" in out
+    got = [(int(m[1]), [int(m[3]), int(m[4]), int(m[5])], [float(v) for v in m[6].split(",")],
+            [float(v) for v in m[7].split(",")]) for m in LEVEL.finditer(out)]
+    spheres = puncture_spheres([0.5, 0.5], [[1.5, 0, 0], [-1.5, 0, 0]], 15)
+    lo = [-12.0 + OFFSET] * 3
+    hi = [12.0 + OFFSET] * 3
+    ref = tracking_levels([24] * 3, lo, hi, spheres, 15)
+    assert len(ref) == 4 and len(got) == 3
+    for (l, n, glo, ghi), lv in zip(got, ref[1:]):
+        (rn, rlo, rhi), = lv
+        assert n == rn
+        # printed with 6 significant digits
+        assert all(abs(a - b) <= 1e-5 * max(1.0, abs(b)) for a, b in zip(glo + ghi, rlo + rhi))
+
+
+def test_dry_run_single_level_gauge_wave():
+    out = run(os.path.join(FIX, "gauge_wave_small.yaml"), "--dry-run", check=True).stdout
+    assert "Gauge wave: A=0.1, L=1" in out
+    assert "Grid: 16 x 8 x 8" in out and "dx = 0.0625, dt = 0.015625" in out
+    assert not LEVEL.search(out)
+
+
+def test_rejects_unsupported_initial_data(tmp_path):
+    p = tmp_path / "tp.yaml"
+    p.write_text("initial_data:\n  type: two_punctures\n")
+    res = run(str(p), "--dry-run")
+    assert res.returncode == 2 and "elliptic" in res.stderr
+
+
+def test_reports_yaml_errors(tmp_path):
+    p = tmp_path / "bad.yaml"
+    p.write_text("grid:\n  ncells: [8, 8\n")
+    res = run(str(p), "--dry-run")
+    assert res.returncode == 1 and "YAML parsing error" in res.stderr
+    p.write_text("grid:\n  ncells: [8, 8, 8]\n  ghost_cells: 3\n")
+    res = run(str(p), "--dry-run")
+    assert res.returncode == 2 and "ghost_cells" in res.stderr

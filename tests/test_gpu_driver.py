@@ -1,0 +1,65 @@
+"""granite_main on the GPU: a small binary Brill-Lindquist run prints the reference's
+telemetry lines, and its numbers are those of the same hierarchy driven through the Python
+binding of the C ABI."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from granite_amd import build as B
+from granite_amd.hierarchy import puncture_spheres, tracking_levels
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OFFSET = 1.3621415e-6
+STEP = re.compile(r"step=(\d+)\s+t=(\S+)\s+α_center=(\S+)\s+\|\|H\|\|₂=(\S+)\s+\[Blocks:\s*(\d+)\]")
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def test_driver_run_matches_binding(engine):
+    import granite_amd as ga
+
+    if not os.path.exists(B.DRIVER):
+        pytest.skip("driver not built")
+    say("granite_main bl_binary_small.yaml")
+    res = subprocess.run([B.DRIVER, os.path.join(HERE, "driver", "bl_binary_small.yaml")], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stderr[-2000:]
+    out = res.stdout
+    assert "Evolution complete." in out
+    for s in range(1, 7):
+        assert f"[NaN@step={s}] all finite" in out
+    rows = [(int(m[1]), float(m[2]), float(m[3]), float(m[4]), int(m[5])) for m in STEP.finditer(out)]
+    assert [r[0] for r in rows] == [2, 4, 6] and all(r[4] == 4 for r in rows)
+
+    # the same run through the binding
+    spheres = puncture_spheres([0.5, 0.5], [[1.5, 0, 0], [-1.5, 0, 0]], 15)
+    geo = tracking_levels([24] * 3, [-12.0 + OFFSET] * 3, [12.0 + OFFSET] * 3, spheres, 15)
+    levels = [ga.Level([ga.GridBlock(0, l, n, lo, hi, 4) for n, lo, hi in lv], engine) for l, lv in enumerate(geo)]
+    H = ga.Hierarchy(levels, engine)
+    levels[0].init_brill_lindquist([0.5, 0.5], [[1.5, 0, 0], [-1.5, 0, 0]])
+    for l in range(1, 4):
+        H.prolongate(l)
+    H.fill_ghosts(0)
+    p = ga.CCZ4Params(kappa1=0.02, kappa2=0.0, eta=2.0, ko_sigma=0.1)
+    mine = []
+    for step in range(1, 7):
+        H.step(p, 0.25, ga.PREC_FP64, use_graph=True)
+        if step % 2 == 0:
+            sh, _, n = H.constraints_l2()
+            mine.append((step, levels[0].read_cell(0, 18, 16, 16, 16), np.sqrt(sh / n)))
+    for (s, t, a, h, _), (s2, a2, h2) in zip(rows, mine):
+        say(f"step {s}: driver alpha_c={a} H={h}  binding alpha_c={a2:.6g} H={h2:.6g}")
+        assert s == s2 and abs(t - 0.25 * s) < 1e-12
+        assert abs(a - a2) <= 1e-5 * abs(a2) and abs(h - h2) <= 1e-5 * h2  # printed to 6 digits
+    H.close()
+    for L in levels:
+        L.close()
