@@ -43,7 +43,7 @@ constexpr double PI = 3.14159265358979323846;
 
 // Order in which ccz4_cell stages the fields (Stager::field calls). A stager may rely on it
 // to prefetch the next fields while the current one is being evaluated.
-constexpr int FIELD_ORDER[NV] = {19, 20, 21, 0, 18, 1, 2, 3, 4, 5, 6, 14, 15, 16, 13, 17, 7, 8, 9, 10, 11, 12};
+constexpr int FIELD_ORDER[NV] = {19, 20, 21, 0, 18, 1, 2, 3, 4, 5, 6, 13, 14, 15, 16, 17, 7, 8, 9, 10, 11, 12};
 __host__ __device__ constexpr int field_pos(int v) {
     int p = 0;
     while (FIELD_ORDER[p] != v) ++p;
@@ -228,7 +228,7 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
 
     // Gamma-hat RHS (ccz4.cpp:747-815) is gathered in GA as its ingredients appear:
     // T5 + T6 from the shift's second derivatives, T1 + T4 + T8 once the Christoffels and
-    // A^i_j exist, T2 after K, T3 from the sources, T7 at the end.
+    // A^i_j exist, T2 and T3 with K, T7 as each Gamma-hat field is stored.
     R GA[3];
 
     // ---- shift: keeps d beta; second derivatives give the Laplacian T5 and the grad-div T6
@@ -298,24 +298,50 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
             pin(chr[i][p]);
         }
 
-    // ---- mixed A^i_j = g^ik A_kj; A~A~ (ccz4.cpp:609-619); T1 + T4 + T8 of Gamma-hat
-    R Au[3][3];
+    // ---- covariant Hessian of the lapse D_i D_j alpha (ccz4.cpp:678-680) and the
+    //      conformal-factor Ricci R^chi (ccz4.cpp:518-541), folded in as soon as the
+    //      Christoffels exist so that the raw second derivatives of chi and alpha die here
+    R DDa[6];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            Au[i][j] = gu[sym(i, 0)] * At[sym(0, j)] + gu[sym(i, 1)] * At[sym(1, j)] + gu[sym(i, 2)] * At[sym(2, j)];
-    R AA = R(0);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) AA += Au[i][j] * Au[j][i];
-    pin(AA);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) pin(Au[i][j]);
+    for (int p = 0; p < 6; ++p) {
+        DDa[p] = ddal[p] - chr[0][p] * dal[0] - chr[1][p] * dal[1] - chr[2][p] * dal[2];
+        pin(DDa[p]);
+    }
     {
+        R lapchi = R(0), gradchi2 = R(0);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int l = sym_a(q), m = sym_b(q);
+            const R f = (l == m) ? R(1) : R(2);
+            lapchi += f * gu[q] * (ddchi[q] - chr[0][q] * dchi[0] - chr[1][q] * dchi[1] - chr[2][q] * dchi[2]);
+            gradchi2 += f * gu[q] * dchi[l] * dchi[m];
+        }
+        const R c1 = R(0.5) / (chi + R(1e-30));
+        const R c2 = R(0.25) / ((chi * chi) + R(1e-30));
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            const int a = sym_a(p), b = sym_b(p);
+            const R Dab = ddchi[p] - chr[0][p] * dchi[0] - chr[1][p] * dchi[1] - chr[2][p] * dchi[2];
+            Rt[p] += c1 * (Dab + gt[p] * lapchi) - c2 * (dchi[a] * dchi[b] - R(3) * gt[p] * gradchi2);
+            pin(Rt[p]);
+        }
+    }
+
+    // ---- A~A~ = A^i_j A^j_i with A^i_j = g^ik A_kj (ccz4.cpp:609-619); T1 + T4 + T8 of
+    //      Gamma-hat. A^i_j is formed here and again for the A~ RHS rather than kept live.
+    R AA = R(0);
+    {
+        R Au[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Au[i][j] = gu[sym(i, 0)] * At[sym(0, j)] + gu[sym(i, 1)] * At[sym(1, j)] + gu[sym(i, 2)] * At[sym(2, j)];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) AA += Au[i][j] * Au[j][i];
+        pin(AA);
         const R c4 = R(3) * alpha / (chi + R(1e-30));
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -328,17 +354,48 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
         }
     }
 
-    // ---- Gamma-hat fields: Ricci term 2, 1/2 (gt_ki d_j Gh^k + gt_kj d_i Gh^k) (ccz4.cpp:480-484)
-    R advGh[3], koGh[3];
+    // ---- K (ccz4.cpp:621-652); its gradient completes T2 of Gamma-hat
+    {
+        R dK[3];
+        eval_field<false, KO_EXACT>(S.field(KTR), K, S.strides(), be, w, dK, dd, adv, ko);
+        R lap = R(0), cga = R(0);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int l = sym_a(q), m = sym_b(q);
+            const R f = (l == m) ? R(1) : R(2);
+            lap += f * gu[q] * DDa[q];
+            cga += gu[q] * (l == m ? dchi[l] * dal[l] : dchi[l] * dal[m] + dchi[m] * dal[l]);
+        }
+        const R plap = chi * lap - R(0.5) * cga;
+        const R matter = HAS_SRC ? R(4) * R(PI) * alpha * (R(src.rho) + R(src.S)) : R(0);
+        O.put(KTR,
+              -plap + alpha * (AA + Ktr * Ktr / R(3)) + matter + R(P.kappa1) * (R(1) - R(P.kappa2)) * alpha * Theta + adv,
+              ko, Ktr);
+        const R c2 = -(R(4) / R(3)) * alpha;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            GA[i] += c2 * (gu[sym(i, 0)] * dK[0] + gu[sym(i, 1)] * dK[1] + gu[sym(i, 2)] * dK[2]);
+            if (HAS_SRC)
+                GA[i] += -R(16) * R(PI) * alpha *
+                    (gu[sym(i, 0)] * R(src.Si[0]) + gu[sym(i, 1)] * R(src.Si[1]) + gu[sym(i, 2)] * R(src.Si[2]));
+            pin(GA[i]);
+        }
+    }
+
+    // ---- Gamma-hat fields: Ricci term 2, 1/2 (gt_ki d_j Gh^k + gt_kj d_i Gh^k) (ccz4.cpp:480-484);
+    //      RHS with T7 = -Gh^j d_j beta^i + advection + 2/3 Gh^i div beta, stored at once
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        eval_field<false, KO_EXACT>(S.field(GHX + q), K, S.strides(), be, w, d1, dd, advGh[q], koGh[q]);
+        R advq, koq;
+        eval_field<false, KO_EXACT>(S.field(GHX + q), K, S.strides(), be, w, d1, dd, advq, koq);
 #pragma unroll
         for (int p = 0; p < 6; ++p) {
             const int a = sym_a(p), b = sym_b(p);
             Rt[p] += R(0.5) * (gt[sym(q, a)] * d1[b] + gt[sym(q, b)] * d1[a]);
             pin(Rt[p]);
         }
+        const R T7 = -(Gh[0] * dbe[q][0] + Gh[1] * dbe[q][1] + Gh[2] * dbe[q][2]) + advq + (R(2) / R(3)) * Gh[q] * divb;
+        O.put(GHX + q, GA[q] + T7, koq, Gh[q]);
     }
 
     // ---- remaining Ricci terms (ccz4.cpp:486-515), factored per k with G_k = gt_kn chr^n:
@@ -367,66 +424,11 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
             }
         }
     }
-
-    // ---- conformal-factor Ricci R^chi (ccz4.cpp:518-541); R_ij = R~_ij + R^chi_ij; and the
-    //      covariant Hessian of the lapse D_i D_j alpha (ccz4.cpp:678-680)
-    R DDa[6];
-    {
-        R lapchi = R(0), gradchi2 = R(0);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int l = sym_a(q), m = sym_b(q);
-            const R f = (l == m) ? R(1) : R(2);
-            lapchi += f * gu[q] * (ddchi[q] - chr[0][q] * dchi[0] - chr[1][q] * dchi[1] - chr[2][q] * dchi[2]);
-            gradchi2 += f * gu[q] * dchi[l] * dchi[m];
-        }
-        const R c1 = R(0.5) / (chi + R(1e-30));
-        const R c2 = R(0.25) / ((chi * chi) + R(1e-30));
-#pragma unroll
-        for (int p = 0; p < 6; ++p) {
-            const int a = sym_a(p), b = sym_b(p);
-            const R Dab = ddchi[p] - chr[0][p] * dchi[0] - chr[1][p] * dchi[1] - chr[2][p] * dchi[2];
-            Rt[p] += c1 * (Dab + gt[p] * lapchi) - c2 * (dchi[a] * dchi[b] - R(3) * gt[p] * gradchi2);
-            DDa[p] = ddal[p] - chr[0][p] * dal[0] - chr[1][p] * dal[1] - chr[2][p] * dal[2];
-            pin(Rt[p]);
-            pin(DDa[p]);
-        }
-    }
     R Rs = R(0);
 #pragma unroll
     for (int q = 0; q < 6; ++q) Rs += ((q == 0 || q == 3 || q == 5) ? R(1) : R(2)) * gu[q] * Rt[q];
     Rs *= chi;
     pin(Rs);
-
-    // ---- K (ccz4.cpp:621-652)
-    R dK[3];
-    eval_field<false, KO_EXACT>(S.field(KTR), K, S.strides(), be, w, dK, dd, adv, ko);
-    {
-        R lap = R(0), cga = R(0);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int l = sym_a(q), m = sym_b(q);
-            const R f = (l == m) ? R(1) : R(2);
-            lap += f * gu[q] * DDa[q];
-            cga += gu[q] * (l == m ? dchi[l] * dal[l] : dchi[l] * dal[m] + dchi[m] * dal[l]);
-        }
-        const R plap = chi * lap - R(0.5) * cga;
-        const R matter = HAS_SRC ? R(4) * R(PI) * alpha * (R(src.rho) + R(src.S)) : R(0);
-        O.put(KTR,
-              -plap + alpha * (AA + Ktr * Ktr / R(3)) + matter + R(P.kappa1) * (R(1) - R(P.kappa2)) * alpha * Theta + adv,
-              ko, Ktr);
-    }
-    {
-        const R c2 = -(R(4) / R(3)) * alpha;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            GA[i] += c2 * (gu[sym(i, 0)] * dK[0] + gu[sym(i, 1)] * dK[1] + gu[sym(i, 2)] * dK[2]);
-            if (HAS_SRC)
-                GA[i] += -R(16) * R(PI) * alpha *
-                    (gu[sym(i, 0)] * R(src.Si[0]) + gu[sym(i, 1)] * R(src.Si[1]) + gu[sym(i, 2)] * R(src.Si[2]));
-            pin(GA[i]);
-        }
-    }
 
     // ---- Theta (ccz4.cpp:654-665)
     eval_field<false, KO_EXACT>(S.field(THETA), K, S.strides(), be, w, d1, dd, adv, ko);
@@ -438,13 +440,6 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
               ko, Theta);
     }
 
-    // ---- Gamma-hat fields' RHS: T7 = -Gh^j d_j beta^i + advection + 2/3 Gh^i div beta
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const R T7 = -(Gh[0] * dbe[i][0] + Gh[1] * dbe[i][1] + Gh[2] * dbe[i][2]) + advGh[i] + (R(2) / R(3)) * Gh[i] * divb;
-        O.put(GHX + i, GA[i] + T7, koGh[i], Gh[i]);
-    }
-
     // ---- A~: raw value, then trace removed with the full raw trace (ccz4.cpp:671-726)
     R raw[6], koA[6];
 #pragma unroll
@@ -453,7 +448,11 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
         eval_field<false, KO_EXACT>(S.field(AXX + p), K, S.strides(), be, w, d1, dd, adv, koA[p]);
         const R Sij = HAS_SRC ? R(src.Sij[p]) : R(0);
         const R t1 = chi * (-DDa[p] + alpha * (Rt[p] - R(8) * R(PI) * Sij));
-        const R Asq = At[sym(a, 0)] * Au[0][b] + At[sym(a, 1)] * Au[1][b] + At[sym(a, 2)] * Au[2][b];
+        R Aub[3]; // A^q_b = g^qr A_rb
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            Aub[q] = gu[sym(q, 0)] * At[sym(0, b)] + gu[sym(q, 1)] * At[sym(1, b)] + gu[sym(q, 2)] * At[sym(2, b)];
+        const R Asq = At[sym(a, 0)] * Aub[0] + At[sym(a, 1)] * Aub[1] + At[sym(a, 2)] * Aub[2];
         const R t2 = alpha * (Ktr * At[p] - R(2) * Asq);
         R L = adv;
 #pragma unroll
