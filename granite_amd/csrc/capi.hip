@@ -144,7 +144,8 @@ const char* gr_version(void) { return kVersion; }
 int gr_stage_tile(int precision, int* t3) {
     if (!t3) return GR_ERR_ARG;
     if (precision != GR_PREC_FP64 && precision != GR_PREC_MIXED) return GR_ERR_ARG;
-    gr::stage_tile_fp64(t3);
+    if (precision == GR_PREC_FP64) gr::stage_tile_fp64(t3);
+    else gr::stage_tile_mixed(t3);
     return GR_OK;
 }
 

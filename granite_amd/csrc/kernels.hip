@@ -11,15 +11,21 @@
 
 #include <cstdint>
 
-// tile shape and prefetch depth of the FP64 stage kernel (overridable at build time)
-#ifndef GR_FP64_TILE_X
-#define GR_FP64_TILE_X 8
+// tile shape (overridable at build time): a tile of GR_TILE_X x Y x Z cells is the unit of
+// the launch bookkeeping and of the mixed-precision classification and kernel (one thread
+// per cell); the FP64 kernel, whose registers allow one wave per SIMD, runs each tile as
+// GR_FP64_SPLIT workgroups of GR_TILE_X / GR_FP64_SPLIT x Y x Z cells
+#ifndef GR_TILE_X
+#define GR_TILE_X 16
 #endif
-#ifndef GR_FP64_TILE_Y
-#define GR_FP64_TILE_Y 4
+#ifndef GR_TILE_Y
+#define GR_TILE_Y 4
 #endif
-#ifndef GR_FP64_TILE_Z
-#define GR_FP64_TILE_Z 8
+#ifndef GR_TILE_Z
+#define GR_TILE_Z 8
+#endif
+#ifndef GR_FP64_SPLIT
+#define GR_FP64_SPLIT 2
 #endif
 #ifndef GR_FP64_PREFETCH
 #define GR_FP64_PREFETCH 1
@@ -484,14 +490,16 @@ __device__ __forceinline__ TileRef resolve_tile(const StageArgs& A, int t) {
     return r;
 }
 
+// part `part` (along x) of tile `tile`: TX x TY x TZ cells starting at x = part * TX
+// inside the tile
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
-__device__ __forceinline__ void stage_tile(const StageArgs& A, int tile, double* lds) {
+__device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R, int part, double* lds) {
     using T = TileGeom<TX, TY, TZ>;
-    const TileRef R = resolve_tile(A, tile);
     const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
-    const int ix = R.tix * TX + lx, iy = R.tiy * TY + ly, iz = R.tiz * TZ + lz; // interior-relative
+    const int x0 = R.tix * GR_TILE_X + part * TX;
+    const int ix = x0 + lx, iy = R.tiy * TY + ly, iz = R.tiz * TZ + lz; // interior-relative
     const bool active = ix < b.n[0] && iy < b.n[1] && iz < b.n[2];
     const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
     const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
@@ -508,7 +516,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, int tile, double*
     io.active = active;
     io.ko_on = A.ko_on != 0;
     io.cf = A.cf;
-    io.init(tid, b.ng + R.tix * TX, b.ng + R.tiy * TY, b.ng + R.tiz * TZ, b.nt[0], b.nt[1], b.nt[2]);
+    io.init(tid, b.ng + x0, b.ng + R.tiy * TY, b.ng + R.tiz * TZ, b.nt[0], b.nt[1], b.nt[2]);
 
     CellSources src{};
     if (HAS_SRC) {
@@ -522,14 +530,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, int tile, double*
     ccz4_cell<double, HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
 }
 
-// one tile per workgroup
+// one part of one tile per workgroup (GR_FP64_SPLIT consecutive workgroups per tile)
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
     using T = TileGeom<TX, TY, TZ>;
     __shared__ double lds[2 * (T::NLDS + 1)];
-    const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
+    const int slot = tile_of_block(blockIdx.x, A.ntiles * GR_FP64_SPLIT, A.xcd_remap);
+    const int tile = slot / GR_FP64_SPLIT, part = slot % GR_FP64_SPLIT;
     if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
-    stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, tile, lds);
+    const TileRef R = resolve_tile(A, tile);
+    if (R.tix * GR_TILE_X + part * TX >= R.b.n[0]) return; // part beyond the block edge
+    stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, R, part, lds);
 }
 
 // the flagged tiles only, from a small grid: each workgroup gathers the tiles with
@@ -551,13 +562,19 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_sc
         const int n = nlist;
         __syncthreads();
         for (int item = 0; item < n; ++item) {
-            stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, list[item], lds);
-            __syncthreads(); // the next tile reuses the LDS buffers
+            const TileRef R = resolve_tile(A, list[item]);
+            for (int part = 0; part < GR_FP64_SPLIT; ++part) {
+                if (R.tix * GR_TILE_X + part * TX >= R.b.n[0]) break; // uniform over the workgroup
+                stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, R, part, lds);
+                __syncthreads(); // the next part reuses the LDS buffers
+            }
         }
     }
 }
 
-constexpr int FP64_TX = GR_FP64_TILE_X, FP64_TY = GR_FP64_TILE_Y, FP64_TZ = GR_FP64_TILE_Z;
+static_assert(GR_TILE_X % GR_FP64_SPLIT == 0, "the FP64 split must divide the tile");
+constexpr int TILE_X = GR_TILE_X, TILE_Y = GR_TILE_Y, TILE_Z = GR_TILE_Z;
+constexpr int FP64_TX = GR_TILE_X / GR_FP64_SPLIT, FP64_TY = GR_TILE_Y, FP64_TZ = GR_TILE_Z;
 // workgroups of the FP64 pass of the mixed form: one per CU
 constexpr int FP64_SCAN_GRID = 256;
 
@@ -1155,6 +1172,12 @@ void stage_tile_fp64(int* t3) {
     t3[2] = FP64_TZ;
 }
 
+void stage_tile_mixed(int* t3) {
+    t3[0] = TILE_X;
+    t3[1] = TILE_Y;
+    t3[2] = TILE_Z;
+}
+
 #ifndef GR_XCD_REMAP
 #define GR_XCD_REMAP 1
 #endif
@@ -1189,8 +1212,7 @@ static StageArgs stage_args(const Batch& B, const int sel[3], const Sources& src
 }
 
 int stage_tiles(const BlockDesc& b) {
-    return ((b.n[0] + FP64_TX - 1) / FP64_TX) * ((b.n[1] + FP64_TY - 1) / FP64_TY) *
-        ((b.n[2] + FP64_TZ - 1) / FP64_TZ);
+    return ((b.n[0] + TILE_X - 1) / TILE_X) * ((b.n[1] + TILE_Y - 1) / TILE_Y) * ((b.n[2] + TILE_Z - 1) / TILE_Z);
 }
 
 static long host_ghost_count(const BlockDesc& b) {
@@ -1204,9 +1226,9 @@ int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, 
     ghost_ofs.assign(1, 0);
     for (size_t i = 0; i < blocks.size(); ++i) {
         BatchBlock& B = blocks[i];
-        B.tiles[0] = (B.b.n[0] + FP64_TX - 1) / FP64_TX;
-        B.tiles[1] = (B.b.n[1] + FP64_TY - 1) / FP64_TY;
-        B.tiles[2] = (B.b.n[2] + FP64_TZ - 1) / FP64_TZ;
+        B.tiles[0] = (B.b.n[0] + TILE_X - 1) / TILE_X;
+        B.tiles[1] = (B.b.n[1] + TILE_Y - 1) / TILE_Y;
+        B.tiles[2] = (B.b.n[2] + TILE_Z - 1) / TILE_Z;
         B.tile0 = t;
         const int nt = B.tiles[0] * B.tiles[1] * B.tiles[2];
         tile_block.insert(tile_block.end(), nt, (int)i);
@@ -1228,9 +1250,9 @@ Batch single_batch(const BlockDesc& b, const double* in, const double* un, doubl
     B.one.buf[1] = const_cast<double*>(un);
     B.one.buf[2] = out;
     B.one.tile0 = 0;
-    B.one.tiles[0] = (b.n[0] + FP64_TX - 1) / FP64_TX;
-    B.one.tiles[1] = (b.n[1] + FP64_TY - 1) / FP64_TY;
-    B.one.tiles[2] = (b.n[2] + FP64_TZ - 1) / FP64_TZ;
+    B.one.tiles[0] = (b.n[0] + TILE_X - 1) / TILE_X;
+    B.one.tiles[1] = (b.n[1] + TILE_Y - 1) / TILE_Y;
+    B.one.tiles[2] = (b.n[2] + TILE_Z - 1) / TILE_Z;
     B.ntiles = B.one.tiles[0] * B.one.tiles[1] * B.one.tiles[2];
     B.nghost = host_ghost_count(b);
     for (int d = 0; d < 3; ++d) B.dx[d] = b.dx[d];
@@ -1243,7 +1265,7 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& sr
     StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 1);
     // with flags (the FP64 tiles of the mixed form: few, near the puncture) a small grid
     // gathers them instead of launching one workgroup per tile of the level
-    int nblk = A.ntiles;
+    int nblk = A.ntiles * GR_FP64_SPLIT;
     if (flags) {
         A.scan = 1;
         nblk = A.ntiles < FP64_SCAN_GRID ? A.ntiles : FP64_SCAN_GRID;
@@ -1261,8 +1283,8 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& sr
 
 template <int MODE, bool HAS_SRC>
 static hipError_t launch_mixed_t(const StageArgs& A, hipStream_t s) {
-    stage_kernel_mixed<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC>
-        <<<dim3(A.ntiles), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
+    stage_kernel_mixed<TILE_X, TILE_Y, TILE_Z, MODE, HAS_SRC>
+        <<<dim3(A.ntiles), dim3(TILE_X * TILE_Y * TILE_Z), 0, s>>>(A);
     return hipGetLastError();
 }
 
@@ -1270,8 +1292,8 @@ hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& s
                               double ko_sigma, int mode, const StageCoef& cf, hipStream_t s, unsigned char* flags,
                               double chi_fp64_below, const AuxStream* aux) {
     const StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 0);
-    classify_tiles_kernel<FP64_TX, FP64_TY, FP64_TZ>
-        <<<dim3(A.ntiles), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A, chi_fp64_below, flags);
+    classify_tiles_kernel<TILE_X, TILE_Y, TILE_Z>
+        <<<dim3(A.ntiles), dim3(TILE_X * TILE_Y * TILE_Z), 0, s>>>(A, chi_fp64_below, flags);
     hipError_t st = hipGetLastError();
     if (st != hipSuccess) return st;
     // the FP64 tiles (flag 1) run beside the FP32 kernel on the auxiliary stream: both read

@@ -131,5 +131,7 @@ hipError_t launch_count_nonfinite(const BlockDesc& b, const double* u, double* p
 
 // tile shape of the FP64 stage kernel
 void stage_tile_fp64(int* t3);
+// tile of the launch bookkeeping and of the mixed-precision kernel (cells x, y, z)
+void stage_tile_mixed(int* t3);
 
 } // namespace gr

@@ -224,7 +224,8 @@ int gr_level_profile_read(gr_level* lvl, int kind, double* total_ms, int* launch
 
 /* ---- introspection ------------------------------------------------------------------- */
 
-/* Stage-kernel tile shape in cells (x, y, z) used for the given precision. */
+/* Stage-kernel tile shape in cells (x, y, z) used for the given precision: FP64, the part of
+ * a tile one workgroup runs; mixed, the whole tile (the unit of the precision choice). */
 int gr_stage_tile(int precision, int* tile3);
 /* Library version string. */
 const char* gr_version(void);
