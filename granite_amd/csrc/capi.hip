@@ -716,6 +716,10 @@ struct gr_hierarchy {
     bool have_graph = false;
     double gkey[8] = {0};
     std::vector<double> policy; // each level's mixed-precision threshold at capture
+    // per level l > 0: prolongation stencil tables of every (fine block f, parent block c)
+    // pair, pair f * nparents + c at ptab_ofs[l][f * nparents + c] entries into ptab[l]
+    std::vector<gr::ProlongAxis*> ptab;
+    std::vector<std::vector<size_t>> ptab_ofs;
 };
 
 namespace {
@@ -745,7 +749,8 @@ hipError_t h_prolongate(gr_hierarchy* H, int l) {
     gr_level* C = H->lv[l - 1];
     for (size_t f = 0; f < F->blk.size(); ++f)
         for (size_t c = 0; c < C->blk.size(); ++c) {
-            hipError_t st = gr::launch_prolongate(C->blk[c], C->u[c], F->blk[f], F->u[f], H->eng->stream);
+            const gr::ProlongAxis* tab = H->ptab[l] + H->ptab_ofs[l][f * C->blk.size() + c];
+            hipError_t st = gr::launch_prolongate(C->blk[c], C->u[c], F->blk[f], F->u[f], tab, H->eng->stream);
             if (st != hipSuccess) return st;
         }
     return hipSuccess;
@@ -819,6 +824,25 @@ int gr_hierarchy_create(gr_engine* e, int nlevels, gr_level* const* levels, gr_h
     H->faces[0].assign(levels[0]->blk.size(), 0x3f);
     for (int l = 1; l < nlevels; ++l)
         for (const gr::BlockDesc& f : levels[l]->blk) H->faces[l].push_back(uncovered_faces(f, levels[l - 1]->blk));
+    H->ptab.assign(nlevels, nullptr);
+    H->ptab_ofs.resize(nlevels);
+    for (int l = 1; l < nlevels; ++l) {
+        size_t n = 0;
+        for (const gr::BlockDesc& f : levels[l]->blk)
+            for (size_t c = 0; c < levels[l - 1]->blk.size(); ++c) {
+                H->ptab_ofs[l].push_back(n);
+                n += (size_t)f.nt[0] + f.nt[1] + f.nt[2];
+            }
+        hipError_t st = hipMalloc((void**)&H->ptab[l], n * sizeof(gr::ProlongAxis));
+        for (size_t f = 0; st == hipSuccess && f < levels[l]->blk.size(); ++f)
+            for (size_t c = 0; st == hipSuccess && c < levels[l - 1]->blk.size(); ++c)
+                st = gr::launch_prolong_axes(levels[l - 1]->blk[c], levels[l]->blk[f],
+                                             H->ptab[l] + H->ptab_ofs[l][f * levels[l - 1]->blk.size() + c], e->stream);
+        if (st != hipSuccess) {
+            gr_hierarchy_destroy(H);
+            return hipfail(e, st, "prolongation stencil tables");
+        }
+    }
     *out = H;
     return GR_OK;
 }
@@ -829,6 +853,8 @@ int gr_hierarchy_destroy(gr_hierarchy* H) {
     (void)hipStreamSynchronize(H->eng->stream);
     if (H->exec) (void)hipGraphExecDestroy(H->exec);
     if (H->graph) (void)hipGraphDestroy(H->graph);
+    for (gr::ProlongAxis* p : H->ptab)
+        if (p) (void)hipFree(p);
     delete H;
     return GR_OK;
 }

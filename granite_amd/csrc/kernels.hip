@@ -48,6 +48,10 @@
 #ifndef GR_LDS_PAD32
 #define GR_LDS_PAD32 1
 #endif
+// field groups (second grid dimension) of the prolongation and restriction kernels
+#ifndef GR_TRANSFER_GROUPS
+#define GR_TRANSFER_GROUPS 4
+#endif
 
 namespace gr {
 
@@ -769,46 +773,112 @@ __global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode, int faces) 
 // AMR transfer between a coarse and a fine block (one block per level)
 // ------------------------------------------------------------------------------------
 
-// Trilinear coarse -> fine into every fine cell, ghosts included, with the reference's
-// index rounding and clamping (src/amr/amr.cpp:484-564).
-__global__ void prolong_kernel(BlockDesc c, const double* __restrict__ cu, BlockDesc f, double* __restrict__ fu) {
+// Per-axis part of the trilinear coarse -> fine stencil, with the reference's index rounding
+// and clamping (src/amr/amr.cpp:484-564): entry e < f.nt[0] is x index e, then the y indices,
+// then the z indices. The stencil is separable, so the prolongation itself reads three entries
+// per fine cell instead of repeating three divisions for every field group.
+__global__ void prolong_axes_kernel(BlockDesc c, BlockDesc f, ProlongAxis* __restrict__ tab) {
 #pragma clang fp contract(off)
-    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); // stride < 2^29
-    if (t >= f.stride) return;
-    const int I[3] = {t % f.nt[0], (t / f.nt[0]) % f.nt[1], t / (f.nt[0] * f.nt[1])};
-    int c0[3], c1[3];
-    double ax[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double x = coord(f, d, I[d]);
-        const double cf = (x - coord(c, d, c.ng)) / c.dx[d] + (double)c.ng;
-        int ci = (int)round(cf);
-        ci = ci < 0 ? 0 : (ci > c.nt[d] - 1 ? c.nt[d] - 1 : ci);
-        const double fr = cf - (double)ci;
-        int cj = ci + (fr >= 0.0 ? 1 : -1);
-        cj = cj > c.nt[d] - 1 ? c.nt[d] - 1 : cj;
-        cj = cj < 0 ? 0 : cj;
-        c0[d] = ci;
-        c1[d] = cj;
-        ax[d] = fabs(fr);
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= f.nt[0] + f.nt[1] + f.nt[2]) return;
+    const int d = t < f.nt[0] ? 0 : (t < f.nt[0] + f.nt[1] ? 1 : 2);
+    const int i = t - (d > 0 ? f.nt[0] : 0) - (d > 1 ? f.nt[1] : 0);
+    const double x = coord(f, d, i);
+    const double cf = (x - coord(c, d, c.ng)) / c.dx[d] + (double)c.ng;
+    int ci = (int)round(cf);
+    ci = ci < 0 ? 0 : (ci > c.nt[d] - 1 ? c.nt[d] - 1 : ci);
+    const double fr = cf - (double)ci;
+    int cj = ci + (fr >= 0.0 ? 1 : -1);
+    cj = cj > c.nt[d] - 1 ? c.nt[d] - 1 : cj;
+    cj = cj < 0 ? 0 : cj;
+    tab[t] = ProlongAxis{ci, cj, fabs(fr)};
+}
+
+// Trilinear coarse -> fine into every fine cell, ghosts included (src/amr/amr.cpp:484-564),
+// from the per-axis stencil table of this (coarse, fine) pair. One workgroup per fine tile of
+// PT_X x PT_Y x PT_Z cells (a thread per (x, y) column of PT_Z cells): per field, the coarse
+// box the tile's stencils reach is staged in LDS (double-buffered, one barrier per field) and
+// the eight taps are read from there, so the coarse field is read from memory once per tile
+// instead of eight times per fine cell. A tile whose stencils reach beyond the box capacity
+// (a fine block not at half the parent's spacing) reads the taps from memory directly.
+constexpr int PT_X = 32, PT_Y = 8, PT_Z = 8;
+constexpr int PB_X = PT_X / 2 + 2, PB_Y = PT_Y / 2 + 2, PB_Z = PT_Z / 2 + 2, PB_N = PB_X * PB_Y * PB_Z;
+
+__global__ __launch_bounds__(PT_X* PT_Y) void prolong_kernel(BlockDesc c, const double* __restrict__ cu, BlockDesc f,
+                                                             double* __restrict__ fu,
+                                                             const ProlongAxis* __restrict__ tab, int ntx, int nty) {
+#pragma clang fp contract(off)
+    __shared__ double box[2][PB_N];
+    __shared__ int lim[6]; // per axis: lowest, highest coarse index any stencil of the tile reads
+    const int tile = blockIdx.x;
+    const int i0 = (tile % ntx) * PT_X, j0 = ((tile / ntx) % nty) * PT_Y, k0 = (tile / (ntx * nty)) * PT_Z;
+    const int tid = threadIdx.x;
+    if (tid < 6) lim[tid] = (tid & 1) ? -1 : 0x7fffffff;
+    __syncthreads();
+    if (tid < PT_X + PT_Y + PT_Z) {
+        const int d = tid < PT_X ? 0 : (tid < PT_X + PT_Y ? 1 : 2);
+        const int e = d == 0 ? i0 + tid : (d == 1 ? j0 + tid - PT_X : k0 + tid - PT_X - PT_Y);
+        if (e < f.nt[d]) {
+            const ProlongAxis A = tab[(d > 0 ? f.nt[0] : 0) + (d > 1 ? f.nt[1] : 0) + e];
+            atomicMin(&lim[2 * d], A.c0 < A.c1 ? A.c0 : A.c1);
+            atomicMax(&lim[2 * d + 1], A.c0 > A.c1 ? A.c0 : A.c1);
+        }
     }
-    const double w000 = (1 - ax[0]) * (1 - ax[1]) * (1 - ax[2]);
-    const double w100 = ax[0] * (1 - ax[1]) * (1 - ax[2]);
-    const double w010 = (1 - ax[0]) * ax[1] * (1 - ax[2]);
-    const double w001 = (1 - ax[0]) * (1 - ax[1]) * ax[2];
-    const double w110 = ax[0] * ax[1] * (1 - ax[2]);
-    const double w101 = ax[0] * (1 - ax[1]) * ax[2];
-    const double w011 = (1 - ax[0]) * ax[1] * ax[2];
-    const double w111 = ax[0] * ax[1] * ax[2];
-    auto at = [&](int a, int b2, int e) { return (long)a + (long)c.nt[0] * ((long)b2 + (long)c.nt[1] * e); };
-    const long p000 = at(c0[0], c0[1], c0[2]), p100 = at(c1[0], c0[1], c0[2]), p010 = at(c0[0], c1[1], c0[2]),
-               p001 = at(c0[0], c0[1], c1[2]), p110 = at(c1[0], c1[1], c0[2]), p101 = at(c1[0], c0[1], c1[2]),
-               p011 = at(c0[0], c1[1], c1[2]), p111 = at(c1[0], c1[1], c1[2]);
-#pragma unroll 4
-    for (int v = 0; v < NV; ++v) {
-        const double* U = cu + v * c.stride;
-        fu[v * f.stride + t] = w000 * U[p000] + w100 * U[p100] + w010 * U[p010] + w001 * U[p001] + w110 * U[p110] +
-            w101 * U[p101] + w011 * U[p011] + w111 * U[p111];
+    __syncthreads();
+    const int bx = lim[0], by = lim[2], bz = lim[4];
+    const int ex = lim[1] - bx + 1, ey = lim[3] - by + 1, ez = lim[5] - bz + 1;
+    const bool staged = ex <= PB_X && ey <= PB_Y && ez <= PB_Z; // uniform over the workgroup
+
+    const int i = i0 + tid % PT_X, j = j0 + tid / PT_X;
+    const bool live = i < f.nt[0] && j < f.nt[1];
+    const ProlongAxis X = tab[live ? i : 0], Y = tab[f.nt[0] + (live ? j : 0)];
+    // leading (x, y) factors of the reference's weight products, (wx * wy) * wz
+    const double p00 = (1 - X.ax) * (1 - Y.ax), p10 = X.ax * (1 - Y.ax), p01 = (1 - X.ax) * Y.ax, p11 = X.ax * Y.ax;
+    const ProlongAxis* Ztab = tab + f.nt[0] + f.nt[1];
+    const int kn = f.nt[2] - k0 < PT_Z ? f.nt[2] - k0 : PT_Z;
+    const int fplane = f.nt[0] * f.nt[1], fcol = i + f.nt[0] * j;
+    if (staged) {
+        const int q00 = (X.c0 - bx) + PB_X * (Y.c0 - by), q10 = (X.c1 - bx) + PB_X * (Y.c0 - by),
+                  q01 = (X.c0 - bx) + PB_X * (Y.c1 - by), q11 = (X.c1 - bx) + PB_X * (Y.c1 - by);
+        int buf = 0;
+        for (int v = blockIdx.y; v < NV; v += gridDim.y) {
+            const double* U = cu + (long)v * c.stride;
+            for (int e = tid; e < PB_N; e += PT_X * PT_Y) {
+                const int a = e % PB_X, b = (e / PB_X) % PB_Y, z = e / (PB_X * PB_Y);
+                if (a < ex && b < ey && z < ez) box[buf][e] = U[(bx + a) + c.nt[0] * ((by + b) + c.nt[1] * (bz + z))];
+            }
+            __syncthreads();
+            if (live) {
+                const double* B = box[buf];
+                double* out = fu + (long)v * f.stride + fcol;
+                for (int kk = 0; kk < kn; ++kk) {
+                    const ProlongAxis Z = Ztab[k0 + kk];
+                    const int z0 = PB_X * PB_Y * (Z.c0 - bz), z1 = PB_X * PB_Y * (Z.c1 - bz);
+                    const double az = Z.ax, bz1 = 1 - Z.ax;
+                    // the reference's term order: 000, 100, 010, 001, 110, 101, 011, 111
+                    out[(long)fplane * (k0 + kk)] = p00 * bz1 * B[q00 + z0] + p10 * bz1 * B[q10 + z0] +
+                        p01 * bz1 * B[q01 + z0] + p00 * az * B[q00 + z1] + p11 * bz1 * B[q11 + z0] +
+                        p10 * az * B[q10 + z1] + p01 * az * B[q01 + z1] + p11 * az * B[q11 + z1];
+                }
+            }
+            buf ^= 1;
+        }
+    } else if (live) {
+        const int r00 = X.c0 + c.nt[0] * Y.c0, r10 = X.c1 + c.nt[0] * Y.c0, r01 = X.c0 + c.nt[0] * Y.c1,
+                  r11 = X.c1 + c.nt[0] * Y.c1;
+        const int cplane = c.nt[0] * c.nt[1];
+        for (int v = blockIdx.y; v < NV; v += gridDim.y) {
+            const double* U = cu + (long)v * c.stride;
+            double* out = fu + (long)v * f.stride + fcol;
+            for (int kk = 0; kk < kn; ++kk) {
+                const ProlongAxis Z = Ztab[k0 + kk];
+                const int z0 = cplane * Z.c0, z1 = cplane * Z.c1;
+                const double az = Z.ax, bz1 = 1 - Z.ax;
+                out[(long)fplane * (k0 + kk)] = p00 * bz1 * U[r00 + z0] + p10 * bz1 * U[r10 + z0] +
+                    p01 * bz1 * U[r01 + z0] + p00 * az * U[r00 + z1] + p11 * bz1 * U[r11 + z0] +
+                    p10 * az * U[r10 + z1] + p01 * az * U[r01 + z1] + p11 * az * U[r11 + z1];
+            }
+        }
     }
 }
 
@@ -827,8 +897,7 @@ __global__ void restrict_kernel(BlockDesc f, const double* __restrict__ fu, Bloc
         if (!(b[d] >= f.ng && b[d] + 2 <= f.ng + f.n[d] + f.ng)) return;
     }
     const long dst = C[0] + (long)c.nt[0] * (C[1] + (long)c.nt[1] * C[2]);
-#pragma unroll 4
-    for (int v = 0; v < NV; ++v) {
+    for (int v = blockIdx.y; v < NV; v += gridDim.y) {
         const double* F = fu + v * f.stride;
         double s = 0.0;
         for (int rk = 0; rk < 2; ++rk)
@@ -1336,14 +1405,21 @@ hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu, hipStream_t s) {
-    prolong_kernel<<<grid_for(f.stride, 256), 256, 0, s>>>(c, cu, f, fu);
+hipError_t launch_prolong_axes(const BlockDesc& c, const BlockDesc& f, ProlongAxis* tab, hipStream_t s) {
+    prolong_axes_kernel<<<grid_for(f.nt[0] + f.nt[1] + f.nt[2], 256), 256, 0, s>>>(c, f, tab);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu,
+                             const ProlongAxis* tab, hipStream_t s) {
+    const int ntx = (f.nt[0] + PT_X - 1) / PT_X, nty = (f.nt[1] + PT_Y - 1) / PT_Y, ntz = (f.nt[2] + PT_Z - 1) / PT_Z;
+    prolong_kernel<<<dim3(ntx * nty * ntz, GR_TRANSFER_GROUPS), PT_X * PT_Y, 0, s>>>(c, cu, f, fu, tab, ntx, nty);
     return hipGetLastError();
 }
 
 hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc& c, double* cu, hipStream_t s) {
     const long n = (long)c.n[0] * c.n[1] * c.n[2];
-    restrict_kernel<<<grid_for(n, 256), 256, 0, s>>>(f, fu, c, cu);
+    restrict_kernel<<<dim3(grid_for(n, 256), GR_TRANSFER_GROUPS), 256, 0, s>>>(f, fu, c, cu);
     return hipGetLastError();
 }
 

@@ -100,9 +100,20 @@ hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream
 // of the faces set in `faces` (bit 2d low side, 2d+1 high side of axis d).
 hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s, int faces = 0x3f);
 
-// trilinear coarse -> fine into every fine cell (ghosts included), and 2x2x2 average of
-// the fine block into the covered coarse interior cells
-hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu, hipStream_t s);
+// one axis of the trilinear coarse -> fine stencil at one fine index: the two coarse
+// neighbours and the weight of c1
+struct ProlongAxis {
+    int c0, c1;
+    double ax;
+};
+
+// stencil table of fine block f inside coarse block c: f.nt[0] + f.nt[1] + f.nt[2] entries
+// (the x indices, then y, then z)
+hipError_t launch_prolong_axes(const BlockDesc& c, const BlockDesc& f, ProlongAxis* tab, hipStream_t s);
+// trilinear coarse -> fine into every fine cell (ghosts included) from that table, and 2x2x2
+// average of the fine block into the covered coarse interior cells
+hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu,
+                             const ProlongAxis* tab, hipStream_t s);
 hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc& c, double* cu, hipStream_t s);
 
 // rhs += Kreiss-Oliger(u) on the interior (reference accumulation order)
