@@ -164,6 +164,33 @@ def run_noisy_192(ga, eng, p, steps):
     return out
 
 
+def checkpoint_staging(ga, eng, levels, H, p, dt0, prec, ms_step):
+    """gr_checkpoint on the hierarchy's whole state (every block, ghosts included, as
+    writeCheckpoint stores it): the device snapshot's time on the engine stream, the
+    device -> pinned host transfer, and coarse steps run while that transfer is in flight."""
+    ck = ga.Checkpoint(levels, eng)
+    nbytes = sum(22 * b.totalSize() * 8 for lv in levels for b in lv.blocks)
+    ck.begin()
+    ck.wait()  # first touch of the pinned pages
+    H.sync()
+    t0 = time.perf_counter()
+    ck.begin()
+    H.sync()
+    t1 = time.perf_counter()
+    ck.wait()
+    t2 = time.perf_counter()
+    ck.begin()
+    for _ in range(3):
+        H.step(p, dt0, prec, use_graph=True)
+    H.sync()
+    t3 = time.perf_counter()
+    ck.wait()
+    ck.close()
+    return {"bytes": nbytes, "snapshot_ms": (t1 - t0) * 1e3, "d2h_ms": (t2 - t1) * 1e3,
+            "d2h_GBps": nbytes / (t2 - t1) / 1e9,
+            "coarse_step_ms_during_transfer": (t3 - t2) * 1e3 / 3, "coarse_step_ms_alone": ms_step}
+
+
 def run_hierarchy(ga, eng, p):
     """The 4-level binary-puncture hierarchy (granite_amd.hierarchy.binary_puncture_levels:
     96^3 on [-64,64]^3 plus tracking-sphere levels of 106x96^2, 114x96^2, 132x96^2 cells at
@@ -197,6 +224,8 @@ def run_hierarchy(ga, eng, p):
         bad = sum(lv.nonfinite() for lv in levels)
         out[name] = {"ms_per_coarse_step": ms, "value": per_step / (ms * 1e-3), "ham_l2_t10": (sh / n) ** 0.5,
                      "mom_l2_t10": (sm / n) ** 0.5, "nonfinite": bad}
+        if name == "fp64":
+            out["checkpoint_staging"] = checkpoint_staging(ga, eng, levels, H, p, dt0, prec, ms)
         H.close()
         for lv in levels:
             lv.close()

@@ -16,6 +16,7 @@ from .ccz4 import (  # noqa: F401
     PREC_MIXED,
     CCZ4Evolution,
     CCZ4Params,
+    Checkpoint,
     Engine,
     EngineError,
     GridBlock,

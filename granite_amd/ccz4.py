@@ -125,6 +125,13 @@ def library():
         "gr_hierarchy_fill_ghosts": (C.c_int, [vp, C.c_int]),
         "gr_hierarchy_step": (C.c_int, [vp, C.POINTER(_Params), C.c_double, C.c_int, C.c_int]),
         "gr_hierarchy_constraints_l2": (C.c_int, [vp, _dp, _dp, C.POINTER(C.c_longlong)]),
+        "gr_checkpoint_create": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(vp)]),
+        "gr_checkpoint_destroy": (C.c_int, [vp]),
+        "gr_checkpoint_begin": (C.c_int, [vp]),
+        "gr_checkpoint_query": (C.c_int, [vp, C.POINTER(C.c_int)]),
+        "gr_checkpoint_wait": (C.c_int, [vp]),
+        "gr_checkpoint_nblocks": (C.c_int, [vp, C.POINTER(C.c_int)]),
+        "gr_checkpoint_block": (C.c_int, [vp, C.c_int, C.POINTER(_dp), C.POINTER(C.c_long)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -472,6 +479,58 @@ class Hierarchy:
     def close(self):
         if getattr(self, "_h", None):
             self._e.lib.gr_hierarchy_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Checkpoint:
+    """Checkpoint staging (gr_checkpoint) for every block of `levels`, coarsest level first:
+    the state HDF5Writer::writeCheckpoint stores (src/io/hdf5_io.cpp:166-232). begin()
+    snapshots the device state in engine-stream order and moves it to pinned host memory on
+    a copy stream while the evolution continues; wait(), then block(i) reads a copy."""
+
+    def __init__(self, levels, engine: Engine | None = None):
+        self._e = engine if engine is not None else Engine.default()
+        self.levels = list(levels)
+        arr = (C.c_void_p * len(self.levels))(*[lv._h for lv in self.levels])
+        h = C.c_void_p()
+        self._e.check(self._e.lib.gr_checkpoint_create(self._e.handle, len(self.levels), arr, C.byref(h)))
+        self._h = h
+        self._geoms = [(lv, b) for lv in self.levels for b in lv.blocks]
+
+    def begin(self):
+        self._e.check(self._e.lib.gr_checkpoint_begin(self._h))
+
+    def done(self) -> bool:
+        d = C.c_int()
+        self._e.check(self._e.lib.gr_checkpoint_query(self._h, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self):
+        self._e.check(self._e.lib.gr_checkpoint_wait(self._h))
+
+    def __len__(self):
+        n = C.c_int()
+        self._e.check(self._e.lib.gr_checkpoint_nblocks(self._h, C.byref(n)))
+        return n.value
+
+    def block(self, index) -> np.ndarray:
+        """Copy of block `index` of the last completed snapshot, shape (22, nz+2g, ny+2g, nx+2g)."""
+        p = _dp()
+        stride = C.c_long()
+        self._e.check(self._e.lib.gr_checkpoint_block(self._h, int(index), C.byref(p), C.byref(stride)))
+        g = self._geoms[index][1]
+        flat = np.ctypeslib.as_array(p, shape=(22 * stride.value,))
+        return flat.reshape(22, g.totalCells(2), g.totalCells(1), g.totalCells(0)).copy()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._e.lib.gr_checkpoint_destroy(self._h)
             self._h = None
 
     def __del__(self):

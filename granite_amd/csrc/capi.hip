@@ -18,6 +18,7 @@
 #endif
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 struct gr_engine {
@@ -956,6 +957,143 @@ int gr_hierarchy_constraints_l2(gr_hierarchy* H, double* ham_sumsq, double* mom_
     if (ham_sumsq) *ham_sumsq = th;
     if (mom_sumsq) *mom_sumsq = tm;
     *count = n;
+    return GR_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------------------------
+// checkpoint staging: device snapshot, then device -> pinned host on a copy stream
+// ------------------------------------------------------------------------------------
+
+struct gr_checkpoint {
+    gr_engine* eng = nullptr;
+    std::vector<gr_level*> lv;
+    std::vector<std::pair<gr_level*, int>> blocks; // (level, block), coarsest level first
+    std::vector<size_t> ofs;                       // first double of each block in the buffers
+    size_t total = 0;                              // doubles
+    double* dev = nullptr;                         // device snapshot
+    double* host = nullptr;                        // pinned host copy
+    hipStream_t copy = nullptr;                    // default priority, like every stream here
+    hipEvent_t taken = nullptr, landed = nullptr;  // snapshot on the engine stream / host copy done
+    bool pending = false;
+};
+
+namespace {
+
+void checkpoint_free(gr_checkpoint* c) {
+    if (c->copy) (void)hipStreamSynchronize(c->copy);
+    if (c->taken) (void)hipEventDestroy(c->taken);
+    if (c->landed) (void)hipEventDestroy(c->landed);
+    if (c->copy) (void)hipStreamDestroy(c->copy);
+    if (c->dev) (void)hipFree(c->dev);
+    if (c->host) (void)hipHostFree(c->host);
+    delete c;
+}
+
+} // namespace
+
+extern "C" {
+
+int gr_checkpoint_create(gr_engine* e, int nlevels, gr_level* const* levels, gr_checkpoint** out) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!out || !levels || nlevels < 1) return fail(e, GR_ERR_ARG, "bad checkpoint arguments");
+    *out = nullptr;
+    for (int l = 0; l < nlevels; ++l)
+        if (!levels[l] || levels[l]->eng != e) return fail(e, GR_ERR_ARG, "every level must belong to this engine");
+    gr_checkpoint* c = new gr_checkpoint();
+    c->eng = e;
+    c->lv.assign(levels, levels + nlevels);
+    for (gr_level* L : c->lv)
+        for (size_t b = 0; b < L->blk.size(); ++b) {
+            c->blocks.emplace_back(L, (int)b);
+            c->ofs.push_back(c->total);
+            c->total += 22 * (size_t)L->blk[b].stride;
+        }
+    hipError_t st = hipMalloc((void**)&c->dev, c->total * sizeof(double));
+    if (st == hipSuccess) st = hipHostMalloc((void**)&c->host, c->total * sizeof(double), hipHostMallocDefault);
+    if (st == hipSuccess) st = hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking);
+    if (st == hipSuccess) st = hipEventCreateWithFlags(&c->taken, hipEventDisableTiming);
+    if (st == hipSuccess) st = hipEventCreateWithFlags(&c->landed, hipEventDisableTiming);
+    if (st != hipSuccess) {
+        checkpoint_free(c);
+        return hipfail(e, st, "checkpoint buffers");
+    }
+    *out = c;
+    return GR_OK;
+}
+
+int gr_checkpoint_destroy(gr_checkpoint* c) {
+    if (!c) return GR_ERR_ARG;
+    (void)hipSetDevice(c->eng->device);
+    checkpoint_free(c);
+    return GR_OK;
+}
+
+int gr_checkpoint_begin(gr_checkpoint* c) {
+    if (!c) return GR_ERR_ARG;
+    gr_engine* e = c->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    // the snapshot may be overwritten only once the previous transfer has read it
+    if (c->pending) GR_HIP(e, hipStreamWaitEvent(e->stream, c->landed, 0));
+    for (size_t i = 0; i < c->blocks.size(); ++i) {
+        gr_level* L = c->blocks[i].first;
+        const int b = c->blocks[i].second;
+        GR_HIP(e, hipMemcpyAsync(c->dev + c->ofs[i], L->u[b], 22 * L->blk[b].stride * sizeof(double),
+                                 hipMemcpyDeviceToDevice, e->stream));
+    }
+    GR_HIP(e, hipEventRecord(c->taken, e->stream));
+    GR_HIP(e, hipStreamWaitEvent(c->copy, c->taken, 0));
+    GR_HIP(e, hipMemcpyAsync(c->host, c->dev, c->total * sizeof(double), hipMemcpyDeviceToHost, c->copy));
+    GR_HIP(e, hipEventRecord(c->landed, c->copy));
+    c->pending = true;
+    return GR_OK;
+}
+
+int gr_checkpoint_query(gr_checkpoint* c, int* done) {
+    if (!c) return GR_ERR_ARG;
+    gr_engine* e = c->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!done) return fail(e, GR_ERR_ARG, "null output");
+    if (!c->pending) {
+        *done = 1;
+        return GR_OK;
+    }
+    const hipError_t st = hipEventQuery(c->landed);
+    if (st == hipErrorNotReady) {
+        *done = 0;
+        return GR_OK;
+    }
+    if (st != hipSuccess) return hipfail(e, st, "hipEventQuery");
+    *done = 1;
+    return GR_OK;
+}
+
+int gr_checkpoint_wait(gr_checkpoint* c) {
+    if (!c) return GR_ERR_ARG;
+    gr_engine* e = c->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (c->pending) GR_HIP(e, hipEventSynchronize(c->landed));
+    return GR_OK;
+}
+
+int gr_checkpoint_nblocks(gr_checkpoint* c, int* nblocks) {
+    if (!c) return GR_ERR_ARG;
+    if (!nblocks) return fail(c->eng, GR_ERR_ARG, "null output");
+    *nblocks = (int)c->blocks.size();
+    return GR_OK;
+}
+
+int gr_checkpoint_block(gr_checkpoint* c, int index, const double** data, long* stride) {
+    if (!c) return GR_ERR_ARG;
+    if (!data || !stride) return fail(c->eng, GR_ERR_ARG, "null output");
+    if (index < 0 || index >= (int)c->blocks.size()) return fail(c->eng, GR_ERR_ARG, "block index out of range");
+    *data = c->host + c->ofs[index];
+    *stride = c->blocks[index].first->blk[c->blocks[index].second].stride;
     return GR_OK;
 }
 

@@ -801,10 +801,23 @@ __global__ void prolong_axes_kernel(BlockDesc c, BlockDesc f, ProlongAxis* __res
 // the eight taps are read from there, so the coarse field is read from memory once per tile
 // instead of eight times per fine cell. A tile whose stencils reach beyond the box capacity
 // (a fine block not at half the parent's spacing) reads the taps from memory directly.
-constexpr int PT_X = 32, PT_Y = 8, PT_Z = 8;
+#ifndef GR_PT_X
+#define GR_PT_X 32
+#endif
+#ifndef GR_PT_Y
+#define GR_PT_Y 8
+#endif
+#ifndef GR_PT_Z
+#define GR_PT_Z 8
+#endif
+#ifndef GR_PROLONG_PREFETCH
+#define GR_PROLONG_PREFETCH 1
+#endif
+constexpr int PT_X = GR_PT_X, PT_Y = GR_PT_Y, PT_Z = GR_PT_Z, PT_N = PT_X * PT_Y;
 constexpr int PB_X = PT_X / 2 + 2, PB_Y = PT_Y / 2 + 2, PB_Z = PT_Z / 2 + 2, PB_N = PB_X * PB_Y * PB_Z;
+constexpr int PB_PER_THREAD = (PB_N + PT_N - 1) / PT_N;
 
-__global__ __launch_bounds__(PT_X* PT_Y) void prolong_kernel(BlockDesc c, const double* __restrict__ cu, BlockDesc f,
+__global__ __launch_bounds__(PT_N) void prolong_kernel(BlockDesc c, const double* __restrict__ cu, BlockDesc f,
                                                              double* __restrict__ fu,
                                                              const ProlongAxis* __restrict__ tab, int ntx, int nty) {
 #pragma clang fp contract(off)
@@ -840,13 +853,42 @@ __global__ __launch_bounds__(PT_X* PT_Y) void prolong_kernel(BlockDesc c, const 
     if (staged) {
         const int q00 = (X.c0 - bx) + PB_X * (Y.c0 - by), q10 = (X.c1 - bx) + PB_X * (Y.c0 - by),
                   q01 = (X.c0 - bx) + PB_X * (Y.c1 - by), q11 = (X.c1 - bx) + PB_X * (Y.c1 - by);
+        // coarse offsets of this thread's box elements (-1: none)
+        int src[PB_PER_THREAD];
+#pragma unroll
+        for (int r = 0; r < PB_PER_THREAD; ++r) {
+            const int e = tid + r * PT_N;
+            const int a = e % PB_X, b = (e / PB_X) % PB_Y, z = e / (PB_X * PB_Y);
+            src[r] = (e < PB_N && a < ex && b < ey && z < ez) ? (bx + a) + c.nt[0] * ((by + b) + c.nt[1] * (bz + z)) : -1;
+        }
+        double next[PB_PER_THREAD];
+#if GR_PROLONG_PREFETCH
+#pragma unroll
+        for (int r = 0; r < PB_PER_THREAD; ++r)
+            if (src[r] >= 0) next[r] = cu[(long)blockIdx.y * c.stride + src[r]];
+#endif
         int buf = 0;
         for (int v = blockIdx.y; v < NV; v += gridDim.y) {
-            const double* U = cu + (long)v * c.stride;
-            for (int e = tid; e < PB_N; e += PT_X * PT_Y) {
-                const int a = e % PB_X, b = (e / PB_X) % PB_Y, z = e / (PB_X * PB_Y);
-                if (a < ex && b < ey && z < ez) box[buf][e] = U[(bx + a) + c.nt[0] * ((by + b) + c.nt[1] * (bz + z))];
+#if GR_PROLONG_PREFETCH
+            // stage field v, then start loading field v + gridDim.y
+#pragma unroll
+            for (int r = 0; r < PB_PER_THREAD; ++r)
+                if (src[r] >= 0) box[buf][tid + r * PT_N] = next[r];
+            if (v + (int)gridDim.y < NV) {
+                const double* Un = cu + (long)(v + gridDim.y) * c.stride;
+#pragma unroll
+                for (int r = 0; r < PB_PER_THREAD; ++r)
+                    if (src[r] >= 0) next[r] = Un[src[r]];
             }
+#else
+            const double* U = cu + (long)v * c.stride;
+#pragma unroll
+            for (int r = 0; r < PB_PER_THREAD; ++r)
+                if (src[r] >= 0) next[r] = U[src[r]];
+#pragma unroll
+            for (int r = 0; r < PB_PER_THREAD; ++r)
+                if (src[r] >= 0) box[buf][tid + r * PT_N] = next[r];
+#endif
             __syncthreads();
             if (live) {
                 const double* B = box[buf];
@@ -1413,7 +1455,7 @@ hipError_t launch_prolong_axes(const BlockDesc& c, const BlockDesc& f, ProlongAx
 hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu,
                              const ProlongAxis* tab, hipStream_t s) {
     const int ntx = (f.nt[0] + PT_X - 1) / PT_X, nty = (f.nt[1] + PT_Y - 1) / PT_Y, ntz = (f.nt[2] + PT_Z - 1) / PT_Z;
-    prolong_kernel<<<dim3(ntx * nty * ntz, GR_TRANSFER_GROUPS), PT_X * PT_Y, 0, s>>>(c, cu, f, fu, tab, ntx, nty);
+    prolong_kernel<<<dim3(ntx * nty * ntz, GR_TRANSFER_GROUPS), PT_N, 0, s>>>(c, cu, f, fu, tab, ntx, nty);
     return hipGetLastError();
 }
 

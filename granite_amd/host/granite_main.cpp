@@ -19,16 +19,28 @@
 //     before the loop and the deeper ones at its first regrids; every fine level is
 //     re-prolongated from its parent at each substep either way) and the hierarchy is not
 //     regridded (the punctures of Brill-Lindquist data do not move);
-//   * no HDF5 output or checkpoints, no CFL-guard line.
-// --dry-run prints the geometry and exits without touching a GPU.
+//   * no HDF5: a checkpoint (step % checkpoint_interval == 0 or t >= t_final, as
+//     src/main.cpp:1256-1261) holds what HDF5Writer::writeCheckpoint stores
+//     (src/io/hdf5_io.cpp:166-232) in a flat little-endian file <output_dir>/checkpoint_<step>.gck:
+//       "GRCKPT01", int32 step, int32 num_blocks, float64 time, then per block (coarsest level
+//       first) int32 level, ng, n[3], nvars, float64 lo[3], dx[3], and nvars * stride float64
+//       (var v = GridBlock::rawData(v), ghosts included). The state is staged through
+//       gr_checkpoint (device snapshot, then a copy stream to pinned memory), so the file is
+//       written while the evolution goes on;
+//   * no CFL-guard line.
+// --dry-run prints the geometry and exits without touching a GPU; --no-checkpoints skips
+// the checkpoint files.
 #include "granite_ccz4.h"
 #include "granite_hip/params_yaml.hpp"
 #include "granite_hip/tracking.hpp"
 
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -154,15 +166,49 @@ std::vector<double> host_initial(const gr_geom& g, const Sim& p) {
     return u;
 }
 
+struct CheckpointFile {
+    std::string path;
+    int step = 0;
+    double time = 0.0;
+};
+
+// the file layout described at the top of this file, from the host views of gr_checkpoint
+void write_checkpoint(gr_checkpoint* c, const CheckpointFile& f, const LevelGeoms& levels, int ng) {
+    std::ofstream out(f.path, std::ios::binary | std::ios::trunc);
+    if (!out) throw std::runtime_error("cannot create " + f.path);
+    auto put = [&](const void* p, size_t n) { out.write(static_cast<const char*>(p), (std::streamsize)n); };
+    int nblocks = 0;
+    if (gr_checkpoint_nblocks(c, &nblocks) != GR_OK) throw std::runtime_error("checkpoint: block count");
+    put("GRCKPT01", 8);
+    const int32_t head[2] = {f.step, nblocks};
+    put(head, sizeof(head));
+    put(&f.time, sizeof(double));
+    int index = 0;
+    for (size_t l = 0; l < levels.size(); ++l)
+        for (const BoxGeom& g : levels[l]) {
+            const double* data = nullptr;
+            long stride = 0;
+            if (gr_checkpoint_block(c, index++, &data, &stride) != GR_OK) throw std::runtime_error("checkpoint: block");
+            const auto dx = g.dx();
+            const int32_t meta[6] = {(int32_t)l, ng, g.n[0], g.n[1], g.n[2], 22};
+            put(meta, sizeof(meta));
+            put(g.lo.data(), 3 * sizeof(double));
+            put(dx.data(), 3 * sizeof(double));
+            put(data, 22 * (size_t)stride * sizeof(double));
+        }
+    if (!out) throw std::runtime_error("write failed: " + f.path);
+}
+
 } // namespace
 
 int main(int argc, char** argv) {
     std::cout << std::unitbuf;
     std::string param_file, precision = "fp64";
-    bool dry = false;
+    bool dry = false, checkpoints = true;
     for (int a = 1; a < argc; ++a) {
         const std::string s = argv[a];
         if (s == "--dry-run") dry = true;
+        else if (s == "--no-checkpoints") checkpoints = false;
         else if (s == "--precision" && a + 1 < argc) precision = argv[++a];
         else param_file = s;
     }
@@ -247,6 +293,7 @@ int main(int argc, char** argv) {
     const int prec = precision == "mixed" ? GR_PREC_MIXED : GR_PREC_FP64;
     std::vector<gr_level*> lv;
     gr_hierarchy* H = nullptr;
+    gr_checkpoint* ck = nullptr;
     try {
         for (const auto& level : levels) {
             std::vector<gr_geom> g;
@@ -273,6 +320,22 @@ int main(int argc, char** argv) {
         }
         for (size_t l = 1; l < lv.size(); ++l) ok(gr_hierarchy_prolongate(H, (int)l));
         ok(gr_hierarchy_fill_ghosts(H, 0));
+
+        if (checkpoints) {
+            std::filesystem::create_directories(p.output_dir); // as HDF5Writer's constructor
+            ok(gr_checkpoint_create(eng, (int)lv.size(), lv.data(), &ck));
+        }
+        bool ck_pending = false;
+        CheckpointFile ck_file;
+        auto flush_checkpoint = [&](bool block) {
+            if (!ck_pending) return;
+            int done = 0;
+            if (block) ok(gr_checkpoint_wait(ck));
+            else ok(gr_checkpoint_query(ck, &done));
+            if (!block && !done) return;
+            write_checkpoint(ck, ck_file, levels, p.ng);
+            ck_pending = false;
+        };
 
         const gr_ccz4_params prm{p.kappa1, p.kappa2, p.eta, p.ko_sigma, 0.05, 0.02};
         const int ic = p.ncells[0] / 2 + p.ng; // the reference's centre probe (src/main.cpp:1238-1244)
@@ -303,16 +366,29 @@ int main(int argc, char** argv) {
                 std::cout << "  step=" << step << "  t=" << t << "  α_center=" << alpha_c << "  ||H||₂=" << hl2
                           << "  [Blocks: " << nblocks << "]\n";
             }
+            flush_checkpoint(false);
+            if (ck && (step % p.checkpoint_interval == 0 || t >= p.t_final)) {
+                flush_checkpoint(true); // the previous file first: begin reuses the host buffers
+                char name[32];
+                std::snprintf(name, sizeof(name), "checkpoint_%06d.gck", step);
+                ck_file = CheckpointFile{p.output_dir + "/" + name, step, t};
+                std::cout << "Writing checkpoint: " << ck_file.path << " (t=" << t << ")\n";
+                ok(gr_checkpoint_begin(ck));
+                ck_pending = true;
+            }
         }
+        flush_checkpoint(true);
         ok(gr_engine_sync(eng));
     } catch (const std::exception& ex) {
         std::cerr << ex.what() << "\n";
+        if (ck) gr_checkpoint_destroy(ck);
         if (H) gr_hierarchy_destroy(H);
         for (gr_level* L : lv) gr_level_destroy(L);
         gr_engine_destroy(eng);
         return 4;
     }
     std::cout << "Evolution complete.\n";
+    if (ck) gr_checkpoint_destroy(ck);
     gr_hierarchy_destroy(H);
     for (gr_level* L : lv) gr_level_destroy(L);
     gr_engine_destroy(eng);

@@ -212,6 +212,34 @@ int gr_hierarchy_step(gr_hierarchy* h, const gr_ccz4_params* prm, double dt0, in
  * sum pointer may be null. */
 int gr_hierarchy_constraints_l2(gr_hierarchy* h, double* ham_sumsq, double* mom_sumsq, long long* count);
 
+/* ---- checkpoint staging ---------------------------------------------------------------------
+ * The data HDF5Writer::writeCheckpoint (src/io/hdf5_io.cpp:166-232) writes at
+ * step % checkpoint_interval == 0 or t >= t_final (src/main.cpp:1256-1261): every block of the
+ * hierarchy, coarsest level first (AMRHierarchy::getAllBlocks), each var v as rawData(v), the
+ * whole stride ghosts included. gr_checkpoint_begin snapshots the device state into a device
+ * copy, ordered on the engine stream after the work already enqueued (a few hundred us for
+ * a GB), and moves that copy into pinned host memory on a copy stream of its own, so the
+ * evolution goes on while the transfer runs. */
+typedef struct gr_checkpoint gr_checkpoint;
+
+/* Snapshot buffers (device and pinned host, 22 * stride doubles per block) for every block of
+ * levels[0..nlevels), in that order. The levels must outlive the checkpoint. */
+int gr_checkpoint_create(gr_engine* eng, int nlevels, gr_level* const* levels, gr_checkpoint** out);
+/* Waits for a pending transfer, then frees the buffers. */
+int gr_checkpoint_destroy(gr_checkpoint* c);
+/* Snapshot every block now (in engine-stream order) and start the transfer; returns without
+ * waiting. A begin while the previous transfer runs is ordered after it on the device; the
+ * caller must be done reading the previous host views. */
+int gr_checkpoint_begin(gr_checkpoint* c);
+/* *done = 1 when the last begin's transfer has completed (0 while it runs). */
+int gr_checkpoint_query(gr_checkpoint* c, int* done);
+/* Blocks until the last begin's transfer has completed. */
+int gr_checkpoint_wait(gr_checkpoint* c);
+/* Number of blocks, and the host view of block `index` (valid after gr_checkpoint_wait until
+ * the next begin): var v is data[v * stride .. v * stride + stride), GridBlock::rawData(v). */
+int gr_checkpoint_nblocks(gr_checkpoint* c, int* nblocks);
+int gr_checkpoint_block(gr_checkpoint* c, int index, const double** data, long* stride);
+
 /* ---- measurement (no reference counterpart) ----------------------------------------------- */
 
 /* enable != 0: every RK3 step run without a graph records HIP events around each of its

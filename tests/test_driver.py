@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from granite_amd import build as B
@@ -65,3 +66,31 @@ def test_reports_yaml_errors(tmp_path):
     p.write_text("grid:\n  ncells: [8, 8, 8]\n  ghost_cells: 3\n")
     res = run(str(p), "--dry-run")
     assert res.returncode == 2 and "ghost_cells" in res.stderr
+
+
+def test_checkpoint_reader_layout(tmp_path):
+    """The reader takes the flat layout granite_main writes: header, then per block its
+    metadata and nvars * stride values (x fastest); a short file is an error."""
+    import struct
+
+    from granite_amd.checkpoint import read_checkpoint
+
+    rng = np.random.default_rng(3)
+    blocks = [(0, 4, (3, 2, 1), rng.standard_normal((22, 9, 10, 11))),
+              (1, 4, (2, 2, 2), rng.standard_normal((22, 10, 10, 10)))]
+    head = b"GRCKPT01" + struct.pack("<iid", 7, len(blocks), 1.75)
+    body = b""
+    for level, ng, n, data in blocks:
+        body += struct.pack("<6i", level, ng, *n, 22) + struct.pack("<6d", -1.0, -2.0, -3.0, 0.5, 0.25, 0.125)
+        body += data.astype("<f8").tobytes()
+    path = tmp_path / "c.gck"
+    path.write_bytes(head + body)
+    ck = read_checkpoint(path)
+    assert ck.step == 7 and ck.time == 1.75 and len(ck.blocks) == 2
+    for got, (level, ng, n, data) in zip(ck.blocks, blocks):
+        assert got.level == level and got.ng == ng and got.n == n
+        assert np.array_equal(got.lo, [-1.0, -2.0, -3.0]) and np.array_equal(got.dx, [0.5, 0.25, 0.125])
+        assert np.array_equal(got.data, data)
+    path.write_bytes(head + body[:-8])
+    with pytest.raises(ValueError):
+        read_checkpoint(path)

@@ -24,14 +24,15 @@ def say(msg):
     sys.stdout.flush()
 
 
-def test_driver_run_matches_binding(engine):
+def test_driver_run_matches_binding(engine, tmp_path):
     import granite_amd as ga
+    from granite_amd.checkpoint import read_checkpoint
 
     if not os.path.exists(B.DRIVER):
         pytest.skip("driver not built")
     say("granite_main bl_binary_small.yaml")
     res = subprocess.run([B.DRIVER, os.path.join(HERE, "driver", "bl_binary_small.yaml")], capture_output=True,
-                         text=True, timeout=120)
+                         text=True, timeout=120, cwd=tmp_path)
     assert res.returncode == 0, res.stderr[-2000:]
     out = res.stdout
     assert "Evolution complete." in out
@@ -60,6 +61,21 @@ def test_driver_run_matches_binding(engine):
         say(f"step {s}: driver alpha_c={a} H={h}  binding alpha_c={a2:.6g} H={h2:.6g}")
         assert s == s2 and abs(t - 0.25 * s) < 1e-12
         assert abs(a - a2) <= 1e-5 * abs(a2) and abs(h - h2) <= 1e-5 * h2  # printed to 6 digits
+
+    # the checkpoint at t = t_final (step 6) holds the binding's final state, bit for bit
+    path = "output/bl_binary_small/checkpoint_000006.gck"
+    assert f"Writing checkpoint: {path} (t=1.5)" in out
+    ck = read_checkpoint(tmp_path / path)
+    assert ck.step == 6 and ck.time == 1.5
+    mine_blocks = [(l, b) for l, L in enumerate(levels) for b in L.blocks]
+    assert len(ck.blocks) == len(mine_blocks)
+    for L in levels:
+        L.download()
+    for cb, (l, b) in zip(ck.blocks, mine_blocks):
+        assert cb.level == l and cb.ng == 4 and cb.n == tuple(b.numCells())
+        assert np.array_equal(cb.dx, [b.dx(d) for d in range(3)])
+        assert np.array_equal(cb.data, b.array), f"level {l}"
+    say(f"checkpoint: {len(ck.blocks)} blocks equal to the binding's state")
     H.close()
     for L in levels:
         L.close()
