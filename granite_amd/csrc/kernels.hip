@@ -123,14 +123,40 @@ __device__ __forceinline__ void algebraic_exact(double g[6], double A[6]) {
 // ------------------------------------------------------------------------------------
 // LDS staging of one field of a tile
 // ------------------------------------------------------------------------------------
-template <int TX, int TY, int TZ, bool PAD = true>
+// Smallest LDS row pitch >= bx0 (elements of esize bytes, rows of a box by rows high) at
+// which the 64 lanes of a wave, laid over the tile x fastest, then y, then z, read their
+// cells without bank conflicts. The LDS serves 64 four-byte banks per pass, so a wave's
+// read of esize-byte elements takes esize / 4 passes of 256 / esize consecutive lanes, each
+// conflict-free when its words fall on distinct banks. The stencil taps shift all lanes
+// alike, so they inherit the pattern of the centre read.
+template <int TX, int TY>
+constexpr int conflict_free_pitch(int bx0, int by, int esize) {
+    const int words = esize / 4, lanes = 64 / words;
+    for (int p = bx0; p < bx0 + 64; ++p) {
+        bool clean = true;
+        for (int pass = 0; pass < words && clean; ++pass) {
+            bool used[64] = {};
+            for (int t = pass * lanes; t < (pass + 1) * lanes && clean; ++t) {
+                const int lx = t % TX, ly = (t / TX) % TY, lz = t / (TX * TY);
+                const int a = (lz * by + ly) * p + lx;
+                for (int w = 0; w < words; ++w) {
+                    const int bank = (a * words + w) % 64;
+                    clean = clean && !used[bank];
+                    used[bank] = true;
+                }
+            }
+        }
+        if (clean) return p;
+    }
+    return bx0;
+}
+
+template <int TX, int TY, int TZ, bool PAD = true, int ESIZE = 8>
 struct TileGeom {
     static constexpr int NT = TX * TY * TZ;
     static constexpr int BX0 = TX + 2 * HALO, BY = TY + 2 * HALO, BZ = TZ + 2 * HALO;
-    // LDS row pitch: a wave covers 8 consecutive rows of one z-plane; a pitch of 8 mod 16
-    // elements puts those rows on disjoint bank ranges (32 or 64 four-byte banks, 4- or
-    // 8-byte elements) instead of all rows starting on the same bank
-    static constexpr int BX = (GR_LDS_PAD && PAD) ? BX0 + ((8 - BX0 % 16) + 16) % 16 : BX0;
+    // LDS row pitch (elements): conflict-free centre reads
+    static constexpr int BX = (GR_LDS_PAD && PAD) ? conflict_free_pitch<TX, TY>(BX0, BY, ESIZE) : BX0;
     static constexpr int NLDS = BX * BY * BZ;
     // sub-boxes of the halo region that some stencil reads (x0,x1,y0,y1,z0,z1)
     static constexpr int NBOX = 7;
@@ -290,7 +316,7 @@ struct TileIO {
 // an O(1) field (1e-8 noise on the metric) keeps FP32 relative precision of the delta.
 template <int TX, int TY, int TZ, int MODE, int PF>
 struct TileIO32 {
-    using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0>;
+    using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0, 4>;
     static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
     static constexpr int LBUF = T::NLDS + 1;
 
@@ -577,6 +603,9 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_sc
 }
 
 static_assert(GR_TILE_X % GR_FP64_SPLIT == 0, "the FP64 split must divide the tile");
+static_assert(!GR_LDS_PAD || GR_TILE_X != 16 || GR_TILE_Y != 4 || GR_FP64_SPLIT != 2 ||
+                  (TileGeom<8, 4, 8>::BX == 24 && TileGeom<16, 4, 8, true, 4>::BX == 48),
+              "LDS pitches of the default tiles: 24 (FP64 rows of 8) and 48 (FP32 rows of 16)");
 constexpr int TILE_X = GR_TILE_X, TILE_Y = GR_TILE_Y, TILE_Z = GR_TILE_Z;
 constexpr int FP64_TX = GR_TILE_X / GR_FP64_SPLIT, FP64_TY = GR_TILE_Y, FP64_TZ = GR_TILE_Z;
 // workgroups of the FP64 pass of the mixed form: one per CU
@@ -584,7 +613,7 @@ constexpr int FP64_SCAN_GRID = 256;
 
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC>
 __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_mixed(StageArgs A) {
-    using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0>;
+    using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0, 4>;
     __shared__ float lds[2 * (T::NLDS + 1)];
 
     const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
