@@ -48,6 +48,10 @@
 #ifndef GR_LDS_PAD32
 #define GR_LDS_PAD32 1
 #endif
+// field groups (second grid dimension) of the ghost fill
+#ifndef GR_GHOST_GROUPS
+#define GR_GHOST_GROUPS 4
+#endif
 // field groups (second grid dimension) of the prolongation and restriction kernels
 #ifndef GR_TRANSFER_GROUPS
 #define GR_TRANSFER_GROUPS 4
@@ -719,7 +723,9 @@ __device__ __forceinline__ long ghost_count(const BlockDesc& b) {
 // (bit 2d: low side of axis d, bit 2d+1: high side); the others keep their values, as
 // fillGhostZones leaves the faces covered by a parent level to prolongation
 // (src/amr/amr.cpp:412-480).
-__device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t_, int mode, int faces = 0x3f) {
+// Fields g, g + GR_GHOST_GROUPS, ... of the cell: the field groups run as separate
+// workgroups (second grid dimension), so a small ghost region still fills the machine.
+__device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t_, int mode, int faces, int g) {
 #pragma clang fp contract(off)
     // ghost cells number fewer than 2^29 (the block limit): 32-bit index arithmetic
     const int t = (int)t_;
@@ -767,10 +773,14 @@ __device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t_, int mode
     }
     const long dst = I[0] + (long)nt0 * (I[1] + (long)nt1 * I[2]);
     const long src = A[0] + (long)nt0 * (A[1] + (long)nt1 * A[2]);
-    // the anchor is never a cell this launch writes, so all 22 loads go out before any store
-    double val[NV];
+    // the anchor is never a cell this launch writes, so all loads go out before any store
+    constexpr int NF = (NV + GR_GHOST_GROUPS - 1) / GR_GHOST_GROUPS;
+    double val[NF];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) val[v] = u[v * b.stride + src];
+    for (int m = 0; m < NF; ++m) {
+        const int v = g + GR_GHOST_GROUPS * m;
+        if (v < NV) val[m] = u[v * b.stride + src];
+    }
     if (mode == 0) {
         // (r_int, r_gh) of each step of the chain: start at the anchor, move one ghost axis
         // at a time in X, Y, Z order
@@ -782,20 +792,23 @@ __device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t_, int mode
             Q[d] = coord(b, d, I[d]);
             const double rgh = radius(Q[0], Q[1], Q[2]);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const double fi = f_inf(v);
-                val[v] = fi + (val[v] - fi) * rin / rgh;
+            for (int m = 0; m < NF; ++m) {
+                const double fi = f_inf(g + GR_GHOST_GROUPS * m);
+                val[m] = fi + (val[m] - fi) * rin / rgh;
             }
         }
     }
 #pragma unroll
-    for (int v = 0; v < NV; ++v) u[v * b.stride + dst] = val[v];
+    for (int m = 0; m < NF; ++m) {
+        const int v = g + GR_GHOST_GROUPS * m;
+        if (v < NV) u[v * b.stride + dst] = val[m];
+    }
 }
 
 __global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode, int faces) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ghost_count(b)) return;
-    fill_ghost_cell(b, u, t, mode, faces);
+    fill_ghost_cell(b, u, t, mode, faces, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------
@@ -990,7 +1003,7 @@ __global__ void fill_ghosts_batch_kernel(const BatchBlock* table, const long* of
         else hi = mid - 1;
     }
     const BatchBlock& B = table[lo];
-    fill_ghost_cell(B.b, B.buf[sel], t - ofs[lo], mode);
+    fill_ghost_cell(B.b, B.buf[sel], t - ofs[lo], mode, 0x3f, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1463,7 +1476,8 @@ hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& s
 
 hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream_t s) {
     if (!B.table) return launch_fill_ghosts(B.one.b, B.one.buf[sel], mode, s, 0x3f);
-    fill_ghosts_batch_kernel<<<grid_for(B.nghost, 256), 256, 0, s>>>(B.table, B.ghost_ofs, B.nblocks, sel, mode);
+    fill_ghosts_batch_kernel<<<dim3(grid_for(B.nghost, 256), GR_GHOST_GROUPS), 256, 0, s>>>(B.table, B.ghost_ofs,
+                                                                                         B.nblocks, sel, mode);
     return hipGetLastError();
 }
 
@@ -1472,7 +1486,7 @@ hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream
     const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
     const long nC = (long)b.n[2] * b.n[1] * 2 * b.ng;
     const long n = nA + nB + nC;
-    fill_ghosts_kernel<<<grid_for(n, 256), 256, 0, s>>>(b, u, mode, faces);
+    fill_ghosts_kernel<<<dim3(grid_for(n, 256), GR_GHOST_GROUPS), 256, 0, s>>>(b, u, mode, faces);
     return hipGetLastError();
 }
 
