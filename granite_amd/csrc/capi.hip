@@ -49,11 +49,12 @@ struct gr_level {
     double* bh = nullptr;      // puncture masses/positions for initial data
     unsigned char* flags = nullptr; // mixed precision: per-tile FP64 flags (all tiles of the level)
     double chi_fp64_below = 0.2;
-    // batched launches over all blocks: device block table, tile -> block map, ghost prefix
+    // batched launches over all blocks: device block table, tile -> block map, ghost
+    // workgroup -> block map
     std::vector<gr::BatchBlock> table;
     gr::BatchBlock* d_table = nullptr;
     int* d_tile_block = nullptr;
-    long* d_ghost_ofs = nullptr;
+    int* d_ghost_wg_block = nullptr;
     gr::Batch batch;
     int partial_cap = 1024;
     // captured RK3 step
@@ -344,8 +345,8 @@ int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** 
         L->table[i].buf[2] = L->s2[i];
     }
     std::vector<int> tile_block;
-    std::vector<long> ghost_ofs;
-    const int ntiles = gr::batch_layout(L->table, tile_block, ghost_ofs);
+    std::vector<int> ghost_wg_block;
+    const int ntiles = gr::batch_layout(L->table, tile_block, ghost_wg_block);
     hipError_t st = hipMalloc((void**)&L->ham, maxstride * sizeof(double));
     L->maxstride = maxstride;
     if (st == hipSuccess) st = hipMalloc((void**)&L->partial, L->partial_cap * sizeof(double));
@@ -353,24 +354,25 @@ int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** 
     if (st == hipSuccess) st = hipMalloc((void**)&L->flags, ntiles);
     if (st == hipSuccess) st = hipMalloc((void**)&L->d_table, nblocks * sizeof(gr::BatchBlock));
     if (st == hipSuccess) st = hipMalloc((void**)&L->d_tile_block, ntiles * sizeof(int));
-    if (st == hipSuccess) st = hipMalloc((void**)&L->d_ghost_ofs, (nblocks + 1) * sizeof(long));
+    if (st == hipSuccess) st = hipMalloc((void**)&L->d_ghost_wg_block, ghost_wg_block.size() * sizeof(int));
     if (st == hipSuccess)
         st = hipMemcpy(L->d_table, L->table.data(), nblocks * sizeof(gr::BatchBlock), hipMemcpyHostToDevice);
     if (st == hipSuccess)
         st = hipMemcpy(L->d_tile_block, tile_block.data(), ntiles * sizeof(int), hipMemcpyHostToDevice);
     if (st == hipSuccess)
-        st = hipMemcpy(L->d_ghost_ofs, ghost_ofs.data(), (nblocks + 1) * sizeof(long), hipMemcpyHostToDevice);
+        st = hipMemcpy(L->d_ghost_wg_block, ghost_wg_block.data(), ghost_wg_block.size() * sizeof(int),
+                       hipMemcpyHostToDevice);
     if (nblocks == 1) {
         L->batch = gr::single_batch(L->blk[0], L->u[0], L->s1[0], L->s2[0]);
     } else {
         gr::Batch& B = L->batch;
         B.table = L->d_table;
         B.tile_block = L->d_tile_block;
-        B.ghost_ofs = L->d_ghost_ofs;
+        B.ghost_wg_block = L->d_ghost_wg_block;
         B.nblocks = nblocks;
         B.one = L->table[0];
         B.ntiles = ntiles;
-        B.nghost = ghost_ofs.back();
+        B.ghost_wgs = (int)ghost_wg_block.size();
         for (int d = 0; d < 3; ++d) B.dx[d] = L->blk[0].dx[d];
     }
     if (st != hipSuccess) {
@@ -401,7 +403,7 @@ int gr_level_destroy(gr_level* L) {
     if (L->flags) (void)hipFree(L->flags);
     if (L->d_table) (void)hipFree(L->d_table);
     if (L->d_tile_block) (void)hipFree(L->d_tile_block);
-    if (L->d_ghost_ofs) (void)hipFree(L->d_ghost_ofs);
+    if (L->d_ghost_wg_block) (void)hipFree(L->d_ghost_wg_block);
     if (L->ham) (void)hipFree(L->ham);
     if (L->mom) (void)hipFree(L->mom);
     if (L->partial) (void)hipFree(L->partial);

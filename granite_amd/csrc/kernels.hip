@@ -725,7 +725,7 @@ __device__ __forceinline__ long ghost_count(const BlockDesc& b) {
 // (src/amr/amr.cpp:412-480).
 // Fields g, g + GR_GHOST_GROUPS, ... of the cell: the field groups run as separate
 // workgroups (second grid dimension), so a small ghost region still fills the machine.
-__device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t_, int mode, int faces, int g) {
+__device__ void fill_ghost_cell(const BlockDesc& b, gdouble* u, long t_, int mode, int faces, int g) {
 #pragma clang fp contract(off)
     // ghost cells number fewer than 2^29 (the block limit): 32-bit index arithmetic
     const int t = (int)t_;
@@ -808,7 +808,7 @@ __device__ void fill_ghost_cell(const BlockDesc& b, double* u, long t_, int mode
 __global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode, int faces) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ghost_count(b)) return;
-    fill_ghost_cell(b, u, t, mode, faces, blockIdx.y);
+    fill_ghost_cell(b, (gdouble*)u, t, mode, faces, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------
@@ -993,17 +993,12 @@ __global__ void restrict_kernel(BlockDesc f, const double* __restrict__ fu, Bloc
 
 // every ghost cell of buffer `sel` of every block of a table; ofs[i] = first ghost cell of
 // block i (nblocks + 1 entries)
-__global__ void fill_ghosts_batch_kernel(const BatchBlock* table, const long* ofs, int nblocks, int sel, int mode) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ofs[nblocks]) return;
-    int lo = 0, hi = nblocks - 1; // largest i with ofs[i] <= t
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ofs[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    const BatchBlock& B = table[lo];
-    fill_ghost_cell(B.b, B.buf[sel], t - ofs[lo], mode, 0x3f, blockIdx.y);
+__global__ void fill_ghosts_batch_kernel(const BatchBlock* __restrict__ table, const int* __restrict__ wg_block,
+                                         int sel, int mode) {
+    const BatchBlock& B = table[wg_block[blockIdx.x]]; // one block per workgroup
+    const long t = (long)(blockIdx.x - B.ghost_wg0) * GHOST_WG + threadIdx.x;
+    if (t >= ghost_count(B.b)) return;
+    fill_ghost_cell(B.b, (gdouble*)B.buf[sel], t, mode, 0x3f, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1372,11 +1367,10 @@ static long host_ghost_count(const BlockDesc& b) {
     return 2L * b.ng * b.nt[1] * b.nt[0] + (long)b.n[2] * 2 * b.ng * b.nt[0] + (long)b.n[2] * b.n[1] * 2 * b.ng;
 }
 
-int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, std::vector<long>& ghost_ofs) {
+int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, std::vector<int>& ghost_wg_block) {
     int t = 0;
-    long g = 0;
     tile_block.clear();
-    ghost_ofs.assign(1, 0);
+    ghost_wg_block.clear();
     for (size_t i = 0; i < blocks.size(); ++i) {
         BatchBlock& B = blocks[i];
         B.tiles[0] = (B.b.n[0] + TILE_X - 1) / TILE_X;
@@ -1386,8 +1380,8 @@ int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, 
         const int nt = B.tiles[0] * B.tiles[1] * B.tiles[2];
         tile_block.insert(tile_block.end(), nt, (int)i);
         t += nt;
-        g += host_ghost_count(B.b);
-        ghost_ofs.push_back(g);
+        B.ghost_wg0 = (int)ghost_wg_block.size();
+        ghost_wg_block.insert(ghost_wg_block.end(), grid_for(host_ghost_count(B.b), GHOST_WG), (int)i);
     }
     return t;
 }
@@ -1396,7 +1390,7 @@ Batch single_batch(const BlockDesc& b, const double* in, const double* un, doubl
     Batch B;
     B.table = nullptr;
     B.tile_block = nullptr;
-    B.ghost_ofs = nullptr;
+    B.ghost_wg_block = nullptr;
     B.nblocks = 1;
     B.one.b = b;
     B.one.buf[0] = const_cast<double*>(in);
@@ -1407,7 +1401,8 @@ Batch single_batch(const BlockDesc& b, const double* in, const double* un, doubl
     B.one.tiles[1] = (b.n[1] + TILE_Y - 1) / TILE_Y;
     B.one.tiles[2] = (b.n[2] + TILE_Z - 1) / TILE_Z;
     B.ntiles = B.one.tiles[0] * B.one.tiles[1] * B.one.tiles[2];
-    B.nghost = host_ghost_count(b);
+    B.one.ghost_wg0 = 0;
+    B.ghost_wgs = (int)grid_for(host_ghost_count(b), GHOST_WG);
     for (int d = 0; d < 3; ++d) B.dx[d] = b.dx[d];
     return B;
 }
@@ -1476,8 +1471,8 @@ hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& s
 
 hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream_t s) {
     if (!B.table) return launch_fill_ghosts(B.one.b, B.one.buf[sel], mode, s, 0x3f);
-    fill_ghosts_batch_kernel<<<dim3(grid_for(B.nghost, 256), GR_GHOST_GROUPS), 256, 0, s>>>(B.table, B.ghost_ofs,
-                                                                                         B.nblocks, sel, mode);
+    fill_ghosts_batch_kernel<<<dim3(B.ghost_wgs, GR_GHOST_GROUPS), GHOST_WG, 0, s>>>(B.table, B.ghost_wg_block, sel,
+                                                                                  mode);
     return hipGetLastError();
 }
 
@@ -1486,7 +1481,7 @@ hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream
     const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
     const long nC = (long)b.n[2] * b.n[1] * 2 * b.ng;
     const long n = nA + nB + nC;
-    fill_ghosts_kernel<<<dim3(grid_for(n, 256), GR_GHOST_GROUPS), 256, 0, s>>>(b, u, mode, faces);
+    fill_ghosts_kernel<<<dim3(grid_for(n, GHOST_WG), GR_GHOST_GROUPS), GHOST_WG, 0, s>>>(b, u, mode, faces);
     return hipGetLastError();
 }
 

@@ -30,26 +30,32 @@ struct BatchBlock {
     double* buf[3];
     int tile0;
     int tiles[3];
+    int ghost_wg0; // first workgroup of the block in a batched ghost launch
 };
 
+// workgroup size of the ghost launches
+constexpr int GHOST_WG = 256;
+
 // What a stage / ghost launch runs over: a device table of blocks (table != nullptr,
-// with tile_block[t] = block of global tile t and ghost_ofs[i] = first ghost cell of
-// block i, nblocks + 1 entries) or the single block `one`. sel_* pick the buffer of
+// with tile_block[t] = block of global tile t and ghost_wg_block[w] = block of ghost
+// workgroup w; each block's ghost cells start on a workgroup of their own) or the single
+// block `one`. sel_* pick the buffer of
 // each block that is the stage input, u^n and the output (0, 1, 2 = buf[0..2]).
 struct Batch {
     const BatchBlock* table;
     const int* tile_block;
-    const long* ghost_ofs;
+    const int* ghost_wg_block;
     int nblocks;
     BatchBlock one;
     int ntiles;
-    long nghost;
+    int ghost_wgs; // workgroups of a ghost launch
     double dx[3]; // grid spacing shared by all blocks of the launch
 };
 
-// builds the tile bookkeeping of a block list (host side): tile0/tiles of each block,
-// the tile->block map and the ghost-cell prefix. Returns the total number of tiles.
-int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, std::vector<long>& ghost_ofs);
+// builds the tile bookkeeping of a block list (host side): tile0/tiles and ghost_wg0 of
+// each block, the tile->block map and the ghost-workgroup->block map. Returns the total
+// number of tiles.
+int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, std::vector<int>& ghost_wg_block);
 // a one-block batch over (in, un, out)
 Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out);
 
