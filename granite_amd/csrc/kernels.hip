@@ -14,7 +14,8 @@
 // tile shape (overridable at build time): a tile of GR_TILE_X x Y x Z cells is the unit of
 // the launch bookkeeping and of the mixed-precision classification and kernel (one thread
 // per cell); the FP64 kernel, whose registers allow one wave per SIMD, runs each tile as
-// GR_FP64_SPLIT workgroups of GR_TILE_X / GR_FP64_SPLIT x Y x Z cells
+// GR_FP64_SPLIT x GR_FP64_ZSPLIT workgroups of GR_TILE_X / GR_FP64_SPLIT x Y x
+// GR_TILE_Z / GR_FP64_ZSPLIT cells
 #ifndef GR_TILE_X
 #define GR_TILE_X 16
 #endif
@@ -23,6 +24,9 @@
 #endif
 #ifndef GR_TILE_Z
 #define GR_TILE_Z 8
+#endif
+#ifndef GR_FP64_ZSPLIT
+#define GR_FP64_ZSPLIT 2
 #endif
 #ifndef GR_FP64_SPLIT
 #define GR_FP64_SPLIT 2
@@ -524,16 +528,17 @@ __device__ __forceinline__ TileRef resolve_tile(const StageArgs& A, int t) {
     return r;
 }
 
-// part `part` (along x) of tile `tile`: TX x TY x TZ cells starting at x = part * TX
-// inside the tile
+// part `part` of a tile: TX x TY x TZ cells at x = (part % GR_FP64_SPLIT) * TX,
+// z = (part / GR_FP64_SPLIT) * TZ inside the tile
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R, int part, double* lds) {
     using T = TileGeom<TX, TY, TZ>;
     const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
-    const int x0 = R.tix * GR_TILE_X + part * TX;
-    const int ix = x0 + lx, iy = R.tiy * TY + ly, iz = R.tiz * TZ + lz; // interior-relative
+    const int x0 = R.tix * GR_TILE_X + (part % GR_FP64_SPLIT) * TX;
+    const int z0 = R.tiz * GR_TILE_Z + (part / GR_FP64_SPLIT) * TZ;
+    const int ix = x0 + lx, iy = R.tiy * TY + ly, iz = z0 + lz; // interior-relative
     const bool active = ix < b.n[0] && iy < b.n[1] && iz < b.n[2];
     const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
     const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
@@ -550,7 +555,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
     io.active = active;
     io.ko_on = A.ko_on != 0;
     io.cf = A.cf;
-    io.init(tid, b.ng + x0, b.ng + R.tiy * TY, b.ng + R.tiz * TZ, b.nt[0], b.nt[1], b.nt[2]);
+    io.init(tid, b.ng + x0, b.ng + R.tiy * TY, b.ng + z0, b.nt[0], b.nt[1], b.nt[2]);
 
     CellSources src{};
     if (HAS_SRC) {
@@ -564,16 +569,26 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
     ccz4_cell<double, HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
 }
 
-// one part of one tile per workgroup (GR_FP64_SPLIT consecutive workgroups per tile)
+// parts of a tile in the FP64 kernels
+constexpr int FP64_PARTS = GR_FP64_SPLIT * GR_FP64_ZSPLIT;
+
+// whether part `part` of tile R holds no cell of the block (uniform over the workgroup)
+template <int TX, int TZ>
+__device__ __forceinline__ bool part_outside(const TileRef& R, int part) {
+    return R.tix * GR_TILE_X + (part % GR_FP64_SPLIT) * TX >= R.b.n[0] ||
+        R.tiz * GR_TILE_Z + (part / GR_FP64_SPLIT) * TZ >= R.b.n[2];
+}
+
+// one part of one tile per workgroup (FP64_PARTS consecutive workgroups per tile)
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
     using T = TileGeom<TX, TY, TZ>;
     __shared__ double lds[2 * (T::NLDS + 1)];
-    const int slot = tile_of_block(blockIdx.x, A.ntiles * GR_FP64_SPLIT, A.xcd_remap);
-    const int tile = slot / GR_FP64_SPLIT, part = slot % GR_FP64_SPLIT;
+    const int slot = tile_of_block(blockIdx.x, A.ntiles * FP64_PARTS, A.xcd_remap);
+    const int tile = slot / FP64_PARTS, part = slot % FP64_PARTS;
     if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
     const TileRef R = resolve_tile(A, tile);
-    if (R.tix * GR_TILE_X + part * TX >= R.b.n[0]) return; // part beyond the block edge
+    if (part_outside<TX, TZ>(R, part)) return;
     stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, R, part, lds);
 }
 
@@ -597,8 +612,8 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_sc
         __syncthreads();
         for (int item = 0; item < n; ++item) {
             const TileRef R = resolve_tile(A, list[item]);
-            for (int part = 0; part < GR_FP64_SPLIT; ++part) {
-                if (R.tix * GR_TILE_X + part * TX >= R.b.n[0]) break; // uniform over the workgroup
+            for (int part = 0; part < FP64_PARTS; ++part) {
+                if (part_outside<TX, TZ>(R, part)) continue; // uniform over the workgroup
                 stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, R, part, lds);
                 __syncthreads(); // the next part reuses the LDS buffers
             }
@@ -606,14 +621,15 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_sc
     }
 }
 
-static_assert(GR_TILE_X % GR_FP64_SPLIT == 0, "the FP64 split must divide the tile");
+static_assert(GR_TILE_X % GR_FP64_SPLIT == 0 && GR_TILE_Z % GR_FP64_ZSPLIT == 0,
+              "the FP64 split must divide the tile");
 static_assert(!GR_LDS_PAD || GR_TILE_X != 16 || GR_TILE_Y != 4 || GR_FP64_SPLIT != 2 ||
                   (TileGeom<8, 4, 8>::BX == 24 && TileGeom<16, 4, 8, true, 4>::BX == 48),
               "LDS pitches of the default tiles: 24 (FP64 rows of 8) and 48 (FP32 rows of 16)");
 constexpr int TILE_X = GR_TILE_X, TILE_Y = GR_TILE_Y, TILE_Z = GR_TILE_Z;
-constexpr int FP64_TX = GR_TILE_X / GR_FP64_SPLIT, FP64_TY = GR_TILE_Y, FP64_TZ = GR_TILE_Z;
-// workgroups of the FP64 pass of the mixed form: one per CU
-constexpr int FP64_SCAN_GRID = 256;
+constexpr int FP64_TX = GR_TILE_X / GR_FP64_SPLIT, FP64_TY = GR_TILE_Y, FP64_TZ = GR_TILE_Z / GR_FP64_ZSPLIT;
+// workgroups of the FP64 pass of the mixed form: 256 threads per CU
+constexpr int FP64_SCAN_GRID = 256 * 256 / (FP64_TX * FP64_TY * FP64_TZ);
 
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC>
 __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_mixed(StageArgs A) {
@@ -1413,7 +1429,7 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& sr
     StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 1);
     // with flags (the FP64 tiles of the mixed form: few, near the puncture) a small grid
     // gathers them instead of launching one workgroup per tile of the level
-    int nblk = A.ntiles * GR_FP64_SPLIT;
+    int nblk = A.ntiles * FP64_PARTS;
     if (flags) {
         A.scan = 1;
         nblk = A.ntiles < FP64_SCAN_GRID ? A.ntiles : FP64_SCAN_GRID;
