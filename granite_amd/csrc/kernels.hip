@@ -469,8 +469,9 @@ struct StageArgs {
 // found in that XCD's L2.
 __device__ __forceinline__ int tile_of_block(int bid, int ntiles, int remap) {
     if (!remap) return bid;
-    const int per = ntiles >> 3;
-    return (bid & 7) * per + (bid >> 3);
+    // XCD x = bid % 8 takes ntiles / 8 consecutive slots, one more for the first ntiles % 8
+    const int q = ntiles >> 3, r = ntiles & 7, x = bid & 7;
+    return x * q + (x < r ? x : r) + (bid >> 3);
 }
 
 // a value the whole workgroup holds alike, moved to scalar registers
@@ -1355,7 +1356,7 @@ static StageArgs stage_args(const Batch& B, const int sel[3], const Sources& src
     A.sel_out = sel[2];
     A.src = src;
     A.ntiles = B.ntiles;
-    A.xcd_remap = (GR_XCD_REMAP && A.ntiles % 8 == 0) ? 1 : 0;
+    A.xcd_remap = GR_XCD_REMAP ? 1 : 0;
     BlockDesc g;
     for (int d = 0; d < 3; ++d) g.dx[d] = B.dx[d];
     A.K = make_stencil(g, ko_sigma);
