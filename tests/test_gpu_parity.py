@@ -267,6 +267,21 @@ def test_bad_arguments_raise(engine):
         ev.computeRHSVacuum(b, r)
     assert ei.value.code == -3
 
+    # level geometry limits, checked before any allocation: a block over 2^29 cells with
+    # ghosts (32-bit byte offsets per field), an empty extent, no blocks at all
+    import ctypes as C
+
+    from granite_amd.ccz4 import _Geom
+
+    lib = engine.lib
+    out = C.c_void_p()
+    for n in ((1024, 1024, 1024), (8, 0, 8)):
+        say(f"error path level of {n}")
+        g = _Geom((C.c_int * 3)(*n), 4, (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(0.1, 0.1, 0.1))
+        assert lib.gr_level_create(engine.handle, 1, C.byref(g), C.byref(out)) == -2
+        assert not out.value
+    assert lib.gr_level_create(engine.handle, 0, None, C.byref(out)) == -1
+
 
 def test_full_size_block_properties(engine):
     """The 128^3 Brill-Lindquist block (dx = 0.25 on [-16,16]^3): after two graph-replayed
