@@ -185,6 +185,22 @@ struct TileGeom {
     __host__ __device__ static constexpr int lds(int x, int y, int z) {
         return ((z + HALO) * BY + (y + HALO)) * BX + (x + HALO);
     }
+    // tile-relative coordinates of halo entry p (boxes in order, x fastest inside a box);
+    // unchanged for p >= TOTAL. The boxes are visited at compile time, so their extents
+    // are immediates and the divisions constant.
+    template <int K = 0>
+    __device__ static void entry(int p, int& x, int& y, int& z) {
+        if constexpr (K < NBOX) {
+            constexpr int nx = box[K][1] - box[K][0], ny = box[K][3] - box[K][2];
+            const int q = p - start(K);
+            if (q >= 0 && q < count(K)) {
+                x = box[K][0] + q % nx;
+                y = box[K][2] + (q / nx) % ny;
+                z = box[K][4] + q / (nx * ny);
+            }
+            entry<K + 1>(p, x, y, z);
+        }
+    }
 };
 
 // Per-thread input/output of one cell of a tile: stages the fields into LDS for the
@@ -225,18 +241,8 @@ struct TileIO {
 #pragma unroll
         for (int i = 0; i < NL; ++i) {
             const int p = tid + i * T::NT;
-            int b = 0, q = p;
-#pragma unroll
-            for (int k = 1; k < T::NBOX; ++k)
-                if (p >= T::start(k)) {
-                    b = k;
-                    q = p - T::start(k);
-                }
-            const int nx = T::box[b][1] - T::box[b][0];
-            const int ny = T::box[b][3] - T::box[b][2];
-            const int x = T::box[b][0] + q % nx;
-            const int y = T::box[b][2] + (q / nx) % ny;
-            const int z = T::box[b][4] + q / (nx * ny);
+            int x = 0, y = 0, z = 0;
+            T::entry(p, x, y, z);
             const int gx = ox + x, gy = oy + y, gz = oz + z;
             const bool ok = p < T::TOTAL && gx < nt0 && gy < nt1 && gz < nt2;
             loff[i] = ok ? T::lds(x, y, z) : T::NLDS;
@@ -351,18 +357,8 @@ struct TileIO32 {
 #pragma unroll
         for (int i = 0; i < NL; ++i) {
             const int p = tid + i * T::NT;
-            int b = 0, q = p;
-#pragma unroll
-            for (int k = 1; k < T::NBOX; ++k)
-                if (p >= T::start(k)) {
-                    b = k;
-                    q = p - T::start(k);
-                }
-            const int nx = T::box[b][1] - T::box[b][0];
-            const int ny = T::box[b][3] - T::box[b][2];
-            const int x = T::box[b][0] + q % nx;
-            const int y = T::box[b][2] + (q / nx) % ny;
-            const int z = T::box[b][4] + q / (nx * ny);
+            int x = 0, y = 0, z = 0;
+            T::entry(p, x, y, z);
             const int gx = ox + x, gy = oy + y, gz = oz + z;
             const bool ok = p < T::TOTAL && gx < nt0 && gy < nt1 && gz < nt2;
             loff[i] = ok ? T::lds(x, y, z) : T::NLDS;
