@@ -620,11 +620,11 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_sc
 
 static_assert(GR_TILE_X % GR_FP64_SPLIT == 0 && GR_TILE_Z % GR_FP64_ZSPLIT == 0,
               "the FP64 split must divide the tile");
-static_assert(!GR_LDS_PAD || GR_TILE_X != 16 || GR_TILE_Y != 4 || GR_FP64_SPLIT != 2 ||
-                  (TileGeom<8, 4, 8>::BX == 24 && TileGeom<16, 4, 8, true, 4>::BX == 48),
-              "LDS pitches of the default tiles: 24 (FP64 rows of 8) and 48 (FP32 rows of 16)");
 constexpr int TILE_X = GR_TILE_X, TILE_Y = GR_TILE_Y, TILE_Z = GR_TILE_Z;
 constexpr int FP64_TX = GR_TILE_X / GR_FP64_SPLIT, FP64_TY = GR_TILE_Y, FP64_TZ = GR_TILE_Z / GR_FP64_ZSPLIT;
+static_assert(!GR_LDS_PAD || FP64_TX != 8 || TILE_X != 16 || TILE_Y != 4 ||
+                  (TileGeom<FP64_TX, FP64_TY, FP64_TZ>::BX == 24 && TileGeom<TILE_X, TILE_Y, TILE_Z, true, 4>::BX == 48),
+              "LDS pitches of the default tiles: 24 (FP64 rows of 8) and 48 (FP32 rows of 16)");
 // workgroups of the FP64 pass of the mixed form: 256 threads per CU
 constexpr int FP64_SCAN_GRID = 256 * 256 / (FP64_TX * FP64_TY * FP64_TZ);
 
