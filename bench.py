@@ -362,6 +362,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": traffic,
+            # HBM-side bytes moved per second at the live launch time (from `traffic`)
+            "traffic_GBs": traffic / (stage_avg_ms * 1e-3) / 1e9 if traffic else None,
             "algorithmic_bytes_per_launch": stage_bytes,
             "avg_launch_ms": stage_avg_ms,
             "launches": n_stage,
