@@ -17,6 +17,7 @@
 #include <array>
 #include <cmath>
 #include <cstddef>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -156,10 +157,101 @@ public:
         eng_->check(gr_level_constraints_l2(l_, &s, &n));
         return n > 0 ? std::sqrt(s / (double)n) : 0.0;
     }
+    gr_level* get() const { return l_; }
 
 private:
     Engine* eng_;
     gr_level* l_ = nullptr;
+};
+
+namespace detail {
+inline std::vector<gr_level*> handles(const std::vector<Level*>& levels) {
+    std::vector<gr_level*> h;
+    for (const Level* l : levels) h.push_back(l->get());
+    return h;
+}
+} // namespace detail
+
+// Device-resident Berger-Oliger hierarchy of fixed geometry, coarsest level first: the
+// operations of amr::AMRHierarchy (include/granite/amr/amr.hpp:49-117) on the device. The
+// levels must outlive it; after a regrid, re-create levels and hierarchy.
+class Hierarchy {
+public:
+    Hierarchy(Engine& engine, const std::vector<Level*>& levels) : eng_(&engine) {
+        const std::vector<gr_level*> h = detail::handles(levels);
+        eng_->check(gr_hierarchy_create(eng_->get(), (int)h.size(), h.data(), &h_));
+    }
+    ~Hierarchy() {
+        if (h_) gr_hierarchy_destroy(h_);
+    }
+    Hierarchy(const Hierarchy&) = delete;
+    Hierarchy& operator=(const Hierarchy&) = delete;
+
+    // setLevelDt(0, dt0) + subcycle(0, sspRK3Step, tagger) without regrid (src/amr/amr.cpp:303-350)
+    template <class Params>
+    void step(const Params& prm, double dt0, int precision = GR_PREC_FP64, bool use_graph = true) {
+        const gr_ccz4_params p = params_of(prm);
+        eng_->check(gr_hierarchy_step(h_, &p, dt0, precision, use_graph ? 1 : 0));
+    }
+    // src/amr/amr.cpp:400-482
+    void fillGhostZones(int level) { eng_->check(gr_hierarchy_fill_ghosts(h_, level)); }
+    // parent level-1 into every cell of level (src/amr/amr.cpp:484-564), as regrid fills a new level
+    void prolongate(int level) { eng_->check(gr_hierarchy_prolongate(h_, level)); }
+    // level into the covered interior of level-1 (src/amr/amr.cpp:566-626)
+    void restrict_data(int level) { eng_->check(gr_hierarchy_restrict(h_, level)); }
+    // ||H||_2 over the interiors of every block of every level (src/main.cpp:1216-1250)
+    double hamiltonianL2() {
+        double s = 0.0;
+        long long n = 0;
+        eng_->check(gr_hierarchy_constraints_l2(h_, &s, nullptr, &n));
+        return n > 0 ? std::sqrt(s / (double)n) : 0.0;
+    }
+
+private:
+    Engine* eng_;
+    gr_hierarchy* h_ = nullptr;
+};
+
+// Checkpoint staging for HDF5Writer::writeCheckpoint (src/io/hdf5_io.cpp:166-232): begin()
+// snapshots every block of the levels (coarsest first) and copies it to pinned host memory
+// while the evolution goes on; after wait(), copyTo() fills a host block (rawData layout).
+class Checkpoint {
+public:
+    Checkpoint(Engine& engine, const std::vector<Level*>& levels) : eng_(&engine) {
+        const std::vector<gr_level*> h = detail::handles(levels);
+        eng_->check(gr_checkpoint_create(eng_->get(), (int)h.size(), h.data(), &c_));
+    }
+    ~Checkpoint() {
+        if (c_) gr_checkpoint_destroy(c_);
+    }
+    Checkpoint(const Checkpoint&) = delete;
+    Checkpoint& operator=(const Checkpoint&) = delete;
+
+    void begin() { eng_->check(gr_checkpoint_begin(c_)); }
+    bool done() {
+        int d = 0;
+        eng_->check(gr_checkpoint_query(c_, &d));
+        return d != 0;
+    }
+    void wait() { eng_->check(gr_checkpoint_wait(c_)); }
+    int numBlocks() {
+        int n = 0;
+        eng_->check(gr_checkpoint_nblocks(c_, &n));
+        return n;
+    }
+    // block i of the last completed snapshot into a host block of the same geometry
+    template <class Block>
+    void copyTo(int i, Block& b) {
+        const double* data = nullptr;
+        long stride = 0;
+        eng_->check(gr_checkpoint_block(c_, i, &data, &stride));
+        if ((std::size_t)stride != b.totalSize()) throw std::runtime_error("granite_hip: checkpoint block size mismatch");
+        std::memcpy(b.flatBuffer(), data, (std::size_t)b.getNumVars() * (std::size_t)stride * sizeof(double));
+    }
+
+private:
+    Engine* eng_;
+    gr_checkpoint* c_ = nullptr;
 };
 
 } // namespace granite_hip

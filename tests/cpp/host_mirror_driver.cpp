@@ -3,10 +3,12 @@
 // Stands in for a GRANITE translation unit: a block type with GridBlock's API (a test double
 // holding the same storage) and a params struct with CCZ4Params' members. Reads a block
 // from a file, runs computeRHSVacuum, computeRHS with zero sources, computeConstraints and
-// two Level::sspRK3Step through the mirror, and writes the results for the caller to check.
+// two Level::sspRK3Step through the mirror, and writes the results for the caller to check;
+// then steps a two-level Hierarchy and checks a Checkpoint against the levels' state.
 //   host_mirror_driver <in.bin> <out.bin>  (in: int n[3], int ng, double lo[3], double hi[3], data)
 #include "granite_hip/ccz4_hip.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -80,6 +82,30 @@ int main(int argc, char** argv) {
         std::fwrite(mom.data()->data(), sizeof(double), 3 * mom.size(), o);
         std::fwrite(s.data.data(), sizeof(double), s.data.size(), o);
         std::fclose(o);
+        // a two-level hierarchy through the mirror (a child at half the spacing inside the
+        // block), one coarse step, then a checkpoint that must equal the levels' state
+        TestBlock c0 = u;
+        TestBlock c1({16, 16, 8}, 4, {-1.0, -1.0, -0.5}, {1.0, 1.0, 0.5});
+        granite_hip::Level L0(eng, std::vector<TestBlock*>{&c0}), L1(eng, std::vector<TestBlock*>{&c1});
+        L0.upload(0, c0);
+        granite_hip::Hierarchy H(eng, {&L0, &L1});
+        H.prolongate(1);
+        H.fillGhostZones(0);
+        H.step(prm, 0.0625);
+        const double h2 = H.hamiltonianL2();
+        granite_hip::Checkpoint ck(eng, {&L0, &L1});
+        ck.begin();
+        ck.wait();
+        TestBlock k0 = c0, k1 = c1;
+        ck.copyTo(0, k0);
+        ck.copyTo(1, k1);
+        L0.download(0, c0);
+        L1.download(0, c1);
+        if (ck.numBlocks() != 2 || k0.data != c0.data || k1.data != c1.data || !std::isfinite(h2)) {
+            std::printf("hierarchy/checkpoint mismatch\n");
+            return 4;
+        }
+        std::printf("hierarchy ok: ||H||2 = %.6e\n", h2);
         // an invalid call must surface as an exception (ng = 3)
         TestBlock bad({4, 4, 4}, 3, {0, 0, 0}, {1, 1, 1}), badr = bad;
         try {

@@ -56,6 +56,7 @@ def test_mirror_against_oracle(driver, oracle, tmp_path):
                          timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "expected error" in out.stdout
+    assert "hierarchy ok" in out.stdout  # Hierarchy + Checkpoint through the mirror
     raw = np.fromfile(tmp_path / "out.bin", dtype=np.float64)
     n = u.size
     N = g.stride
