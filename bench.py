@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector
 N = 128
 HALF = 16.0
 DT = 0.25 * (2 * HALF / N)
@@ -323,9 +324,30 @@ def main():
     final_bytes = 528 * cells
     achieved = stage_bytes / (stage_avg_ms * 1e-3) / 1e9
     traffic = None
+    compute = None
     tpath = os.path.join(ROOT, "profiles", "stage_kernel_traffic_bl128.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        prof = json.load(open(tpath))
+        traffic = prof.get("hbm_bytes_per_launch")
+        if prof.get("fp64_flop_per_launch") and prof.get("cells_per_launch") == cells:
+            # the kernel's own FP64 arithmetic (rocprofv3 instruction counters of this
+            # workload) at the live launch time, against the FP64 vector peak; the issue
+            # floors are the launch times with every SIMD issuing a VALU instruction
+            # (4 cycles per wave64 instruction) every cycle
+            flop = prof["fp64_flop_per_launch"]
+            tf = flop / (stage_avg_ms * 1e-3) / 1e12
+            compute = {
+                "bound": "valu",
+                "achieved": tf,
+                "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s",
+                "frac": tf / PEAK_FP64_TFLOPS,
+                "flop_per_cell": flop / cells,
+                "valu_issue_floor_ms": prof["valu_issue_floor_ms"],
+                "fp64_issue_floor_ms": prof["fp64_issue_floor_ms"],
+                "valu_issue_frac": prof["valu_issue_floor_ms"] / stage_avg_ms,
+                "hbm_frac_at_valu_floor": stage_bytes / (prof["valu_issue_floor_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            }
 
     cpu = {"value": None, "unit": "cell-updates/s", "cores": 0, "kind": None, "sample": "skipped (--no-cpu)"}
     if not args.no_cpu:
@@ -368,6 +390,7 @@ def main():
             "avg_launch_ms": stage_avg_ms,
             "launches": n_stage,
         },
+        "compute": compute,
         "cpu_baseline": cpu,
         "step_algorithmic_GBs": BYTES_PER_CELL_STEP * value / 1e9,
         "kernels_ms": {
