@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -60,3 +61,12 @@ def test_library_contains_gfx950_code(tmp_path):
     if out.returncode != 0:
         pytest.skip("objcopy not available")
     assert b"gfx950" in sec.read_bytes()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU is measured; a request for more exits with a message before touching a device
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr
+    assert out.stdout == ""
