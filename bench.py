@@ -44,6 +44,14 @@ def cpu_baseline():
                 "sample": "failed: " + out.stderr[-300:]}
     res = json.loads(out.stdout.strip().splitlines()[-1])
     res.pop("seconds", None)
+    # the same code on one thread, on a 48^3 block at the same spacing (one RHS evaluation)
+    env1 = dict(env, OMP_NUM_THREADS="1")
+    one = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--n", "48", "--half-width",
+                          str(HALF * 48 / N), "--reps", "1"], capture_output=True, text=True, env=env1, timeout=600,
+                         cwd=ROOT)
+    if one.returncode == 0:
+        r1 = json.loads(one.stdout.strip().splitlines()[-1])
+        res["single_thread"] = {"value": r1["value"], "cores": r1["cores"], "sample": r1["sample"]}
     return res
 
 
