@@ -50,8 +50,9 @@ def main():
             ref.rhs(g, u, rhs)
         dt = (time.perf_counter() - t0) * 3 / args.reps
         kind = "reference"
+        scaled = "" if args.reps == 3 else ", time scaled to 3"
         sample = (f"{args.reps} x computeRHS+applyDissipation of the compiled reference on the {n}^3 Brill-Lindquist "
-                  f"block, scaled to 3 (the RHS work of one RK3 step; combine/floor/ghost passes excluded)")
+                  f"block{scaled} (the RHS work of one RK3 step; combine/floor/ghost passes excluded)")
     else:
         stage = g.zeros()
         rhs = g.zeros()
