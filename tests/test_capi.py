@@ -70,3 +70,15 @@ def test_bench_refuses_more_than_one_gpu():
     assert out.returncode == 2
     assert "one GPU" in out.stderr
     assert out.stdout == ""
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present: the engine can be created")
+def test_no_device_fails_loudly(lib):
+    # without a gfx950 device the engine is not created and nothing falls back to the CPU
+    eng = ctypes.c_void_p()
+    assert lib.gr_engine_create(0, ctypes.byref(eng)) == -4  # GR_ERR_NODEV
+    assert eng.value is None
+    import granite_amd as ga
+
+    with pytest.raises(ga.EngineError):
+        ga.Engine(0)
