@@ -33,21 +33,52 @@ DT = 0.25 * (2 * HALF / N)
 BYTES_PER_CELL_STEP = 1408  # 3 stages x (22-field read + write) + 2 x 22-field read of u^n, FP64
 
 
+def host_cpu():
+    """The host's CPU as lscpu and the scheduler report it."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": None,
+            "physical_cores": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        if kv.get("Core(s) per socket") and kv.get("Socket(s)"):
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def cpu_baseline():
-    threads = min(16, len(os.sched_getaffinity(0)))
+    """The compiled reference (oracle/_ref, Release flags) on this host's cores with the
+    reference launcher's OpenMP placement (scripts/run_granite.py:138-149): every core of the
+    process's affinity, but no more than the CPU share a scheduler grants the job through
+    OMP_NUM_THREADS when it sets it; the median of 3 runs of 5 steps' RHS work, plus a
+    single-thread leg."""
+    host = host_cpu()
+    threads = host["affinity"]
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and 0 < int(share) < threads:
+        threads = int(share)
     env = dict(os.environ)
     env.update(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
     out = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--n", str(N), "--half-width", str(HALF)],
                          capture_output=True, text=True, env=env, timeout=900, cwd=ROOT)
     if out.returncode != 0:
         return {"value": None, "unit": "cell-updates/s", "cores": threads, "kind": None,
-                "sample": "failed: " + out.stderr[-300:]}
+                "sample": "failed: " + out.stderr[-300:], "host": host}
     res = json.loads(out.stdout.strip().splitlines()[-1])
-    res.pop("seconds", None)
-    # the same code on one thread, on a 48^3 block at the same spacing (one RHS evaluation)
+    res["host"] = host
+    res["omp"] = {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores",
+                  "share_from_env": share}
+    # the same code on one thread, on a 48^3 block at the same spacing (median of 3 runs of one
+    # step's RHS work)
     env1 = dict(env, OMP_NUM_THREADS="1")
     one = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--n", "48", "--half-width",
-                          str(HALF * 48 / N), "--reps", "1"], capture_output=True, text=True, env=env1, timeout=600,
+                          str(HALF * 48 / N), "--steps", "1"], capture_output=True, text=True, env=env1, timeout=600,
                          cwd=ROOT)
     if one.returncode == 0:
         r1 = json.loads(one.stdout.strip().splitlines()[-1])

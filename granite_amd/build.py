@@ -48,11 +48,21 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
 
 DRIVER_SRC = os.path.join(HERE, "host", "granite_main.cpp")
 DRIVER = os.path.join(HERE, "bin", "granite_main")
+# where the reference's launcher looks for the binary (scripts/run_granite.py:292-297, with its
+# default build directory "build"), so that launcher runs this driver unchanged from the repo root
+DRIVER_LAUNCHER = os.path.join(ROOT, "build", "bin", "granite_main")
 
 
 def build_driver(verbose: bool = False, lib: str = LIB, out: str = DRIVER) -> str:
     """The granite_main driver (host C++17 over the C ABI), linked to the in-tree library
-    (found at run time through $ORIGIN, so the tree can move)."""
+    (found at run time through $ORIGIN, so the tree can move). Built at `out` and, for the
+    default output, also at build/bin/granite_main."""
+    if out == DRIVER:
+        _link_driver(verbose, lib, DRIVER_LAUNCHER)
+    return _link_driver(verbose, lib, out)
+
+
+def _link_driver(verbose: bool, lib: str, out: str) -> str:
     os.makedirs(os.path.dirname(out), exist_ok=True)
     cxx = os.environ.get("CXX", "g++")
     tmp = out + ".tmp"

@@ -118,6 +118,13 @@ def library():
         "gr_level_set_mixed_policy": (C.c_int, [vp, C.c_double]),
         "gr_level_rhs": (C.c_int, [vp, C.c_int, C.POINTER(_Params), C.c_int, _dp]),
         "gr_level_profile_read": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_int)]),
+        "gr_level_init_gauge_wave": (C.c_int, [vp, C.c_double, C.c_double]),
+        "gr_level_set_sources": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
+        "gr_level_rk3_stage": (C.c_int, [vp, C.POINTER(_Params), C.c_double, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "gr_level_first_nonfinite": (C.c_int, [vp] + [C.POINTER(C.c_int)] * 6 + [_dp]),
+        "gr_level_cfl": (C.c_int, [vp, C.c_double, _dp]),
+        "gr_hierarchy_set_cfl_monitor": (C.c_int, [vp, C.c_int]),
+        "gr_hierarchy_cfl": (C.c_int, [vp, _dp, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
         "gr_hierarchy_create": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(vp)]),
         "gr_hierarchy_destroy": (C.c_int, [vp]),
         "gr_hierarchy_prolongate": (C.c_int, [vp, C.c_int]),
@@ -381,6 +388,34 @@ class Level:
     def fill_ghosts(self, mode):
         self._e.check(self._e.lib.gr_level_fill_ghosts(self._h, int(mode)))
 
+    def init_gauge_wave(self, amplitude, wavelength):
+        self._e.check(self._e.lib.gr_level_init_gauge_wave(self._h, float(amplitude), float(wavelength)))
+
+    def set_sources(self, i, d_rho=None, d_Si=None, d_Sij=None, d_S=None):
+        """Device source arrays (addresses as ints, e.g. torch tensors' data_ptr()) of block i;
+        all None = vacuum."""
+        self._e.check(self._e.lib.gr_level_set_sources(self._h, int(i), d_rho, d_Si, d_Sij, d_S))
+
+    def rk3_stage(self, params: CCZ4Params, dt, stage, precision=PREC_FP64, bc_mode=BC_SOMMERFELD,
+                  pre_fill=BC_NONE):
+        self._e.check(self._e.lib.gr_level_rk3_stage(self._h, C.byref(params.c()), float(dt), int(stage),
+                                                     int(precision), int(bc_mode), int(pre_fill)))
+
+    def first_nonfinite(self):
+        """None, or (block, var, i, j, k, value) of the first non-finite interior value in the
+        reference's scan order (cells counted with ghosts)."""
+        out = [C.c_int() for _ in range(6)]
+        v = C.c_double()
+        self._e.check(self._e.lib.gr_level_first_nonfinite(self._h, *[C.byref(o) for o in out], C.byref(v)))
+        if not out[0].value:
+            return None
+        return tuple(o.value for o in out[1:]) + (v.value,)
+
+    def cfl(self, dt):
+        v = C.c_double()
+        self._e.check(self._e.lib.gr_level_cfl(self._h, float(dt), C.byref(v)))
+        return v.value
+
     def rk3_step(self, params: CCZ4Params, dt, precision=PREC_FP64, bc_mode=BC_SOMMERFELD, pre_fill=BC_NONE,
                  use_graph=False):
         self._e.check(self._e.lib.gr_level_rk3_step(self._h, C.byref(params.c()), float(dt), int(precision),
@@ -472,6 +507,18 @@ class Hierarchy:
         n = C.c_longlong()
         self._e.check(self._e.lib.gr_hierarchy_constraints_l2(self._h, C.byref(h), C.byref(m), C.byref(n)))
         return h.value, m.value, n.value
+
+    def set_cfl_monitor(self, enable=True):
+        self._e.check(self._e.lib.gr_hierarchy_set_cfl_monitor(self._h, 1 if enable else 0))
+
+    def cfl(self):
+        """[(level, advective CFL)] of the last coarse step, one per level step in subcycle order."""
+        n = C.c_int()
+        cap = 1 << 12
+        vals = (C.c_double * cap)()
+        lev = (C.c_int * cap)()
+        self._e.check(self._e.lib.gr_hierarchy_cfl(self._h, vals, lev, cap, C.byref(n)))
+        return [(lev[i], vals[i]) for i in range(n.value)]
 
     def sync(self):
         self._e.check(self._e.lib.gr_engine_sync(self._e.handle))

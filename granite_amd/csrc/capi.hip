@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -37,31 +38,49 @@ struct gr_engine {
     size_t cap_aux = 0;
 };
 
+// Blocks of a level that share one grid spacing bit for bit: one stage launch (the stencil
+// constants are launch arguments). Blocks whose dx differ (by an ulp, as separately built
+// boxes of one level may) run in separate launches with their own constants, as the
+// reference evaluates each block with its own dx.
+struct gr_group {
+    std::vector<int> blocks;         // level block indices, in level order
+    std::vector<gr::BatchBlock> table;
+    gr::BatchBlock* d_table = nullptr;
+    int* d_tile_block = nullptr;
+    int* d_ghost_wg_block = nullptr;
+    unsigned char* flags = nullptr;  // mixed precision: per-tile FP64 flags of the group's tiles
+    gr::Batch batch;
+};
+
 struct gr_level {
     gr_engine* eng = nullptr;
     std::vector<gr::BlockDesc> blk;
     std::vector<double*> u, s1, s2;
+    std::vector<gr::Sources> src;   // per block (device pointers borrowed from the caller)
+    bool has_src = false;
     double* ham = nullptr;     // monitor scratch (max stride)
     double* mom = nullptr;     // momentum-constraint scratch (3 * max stride, on first use)
     long maxstride = 0;
     double* partial = nullptr; // reduction partials
     double* red = nullptr;     // reduction result (1 double)
     double* bh = nullptr;      // puncture masses/positions for initial data
-    unsigned char* flags = nullptr; // mixed precision: per-tile FP64 flags (all tiles of the level)
     double chi_fp64_below = 0.2;
-    // batched launches over all blocks: device block table, tile -> block map, ghost
-    // workgroup -> block map
+    unsigned generation = 0;   // bumped whenever captured launches of the level go stale
+    // batched launches over all blocks (ghost fills, which use each block's own dx): device
+    // block table, tile -> block map, ghost workgroup -> block map
     std::vector<gr::BatchBlock> table;
     gr::BatchBlock* d_table = nullptr;
     int* d_tile_block = nullptr;
     int* d_ghost_wg_block = nullptr;
     gr::Batch batch;
+    // stage launches: one per group of blocks with equal dx
+    std::vector<gr_group> groups;
     int partial_cap = 1024;
     // captured RK3 step
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool have_graph = false;
-    double gkey[10] = {0};
+    double gkey[11] = {0};
     // optional per-kernel event timing of eager steps: kind -> list of (start, stop)
     bool profile = false;
     std::vector<hipEvent_t> ev[3];
@@ -231,8 +250,9 @@ int gr_ccz4_rhs(gr_engine* e, const gr_geom* geom, const gr_ccz4_params* prm, in
     GR_HIP(e, hipMemcpyAsync(e->d_u, u, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
     GR_HIP(e, hipMemcpyAsync(e->d_r, rhs, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
     static const int sel[3] = {0, 1, 2};
-    GR_HIP(e, gr::launch_stage_fp64(gr::single_batch(b, e->d_u, e->d_u, e->d_r), sel, src, cell_params(prm),
-                                    prm->ko_sigma, gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0}, e->stream));
+    GR_HIP(e, gr::launch_stage_fp64(gr::single_batch(b, e->d_u, e->d_u, e->d_r, src), sel, src.rho != nullptr,
+                                    cell_params(prm), prm->ko_sigma, gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0},
+                                    e->stream));
     GR_HIP(e, hipMemcpyAsync(rhs, e->d_r, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     GR_HIP(e, hipStreamSynchronize(e->stream));
     return GR_OK;
@@ -251,8 +271,9 @@ int gr_ccz4_rhs_device(gr_engine* e, const gr_geom* geom, const gr_ccz4_params* 
     if (nsrc != 0 && nsrc != 4) return fail(e, GR_ERR_ARG, "matter sources must be all null or all non-null");
     gr::Sources src{d_rho, d_Si, d_Sij, d_S};
     static const int sel[3] = {0, 1, 2};
-    GR_HIP(e, gr::launch_stage_fp64(gr::single_batch(b, d_u, d_u, d_rhs), sel, src, cell_params(prm), prm->ko_sigma,
-                                    gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0}, e->stream));
+    GR_HIP(e, gr::launch_stage_fp64(gr::single_batch(b, d_u, d_u, d_rhs, src), sel, src.rho != nullptr,
+                                    cell_params(prm), prm->ko_sigma, gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0},
+                                    e->stream));
     return GR_OK;
 }
 
@@ -265,7 +286,7 @@ int gr_ccz4_dissipation(gr_engine* e, const gr_geom* geom, const gr_ccz4_params*
     if (!prm) return fail(e, GR_ERR_ARG, "null parameters");
     if (nvars != GR_NUM_VARS) return fail(e, GR_ERR_NVARS, "nvars must be 22");
     if (!u || !rhs) return fail(e, GR_ERR_ARG, "null state or rhs");
-    if (!(prm->ko_sigma > 0.0)) return GR_OK; // src/spacetime/ccz4.cpp:873-874
+    if (prm->ko_sigma <= 0.0) return GR_OK; // src/spacetime/ccz4.cpp:873-874 (a NaN sigma dissipates)
     const size_t n = 22 * (size_t)b.stride;
     if ((rc = grow(e, &e->d_aux, &e->cap_aux, 2 * n))) return rc;
     double* du = e->d_aux;
@@ -301,6 +322,50 @@ int gr_ccz4_constraints(gr_engine* e, const gr_geom* geom, const double* u, doub
 // levels
 // ---------------------------------------------------------------------------------------
 
+} // extern "C"
+
+// The batched launch description of blocks `which` of a level: host table (geometry, the
+// three buffers, sources), its device copy, the tile -> block and ghost workgroup -> block
+// maps; one block needs no table.
+static hipError_t build_batch(gr_level* L, const std::vector<int>& which, std::vector<gr::BatchBlock>& table,
+                              gr::BatchBlock** d_table, int** d_tile_block, int** d_ghost_wg_block, gr::Batch* B) {
+    table.resize(which.size());
+    for (size_t n = 0; n < which.size(); ++n) {
+        const int i = which[n];
+        table[n].b = L->blk[i];
+        table[n].buf[0] = L->u[i];
+        table[n].buf[1] = L->s1[i];
+        table[n].buf[2] = L->s2[i];
+        table[n].src = L->src[i];
+    }
+    std::vector<int> tile_block, ghost_wg_block;
+    const int ntiles = gr::batch_layout(table, tile_block, ghost_wg_block);
+    if (which.size() == 1) {
+        *B = gr::single_batch(table[0].b, table[0].buf[0], table[0].buf[1], table[0].buf[2], table[0].src);
+        return hipSuccess;
+    }
+    hipError_t st = hipMalloc((void**)d_table, table.size() * sizeof(gr::BatchBlock));
+    if (st == hipSuccess) st = hipMalloc((void**)d_tile_block, ntiles * sizeof(int));
+    if (st == hipSuccess) st = hipMalloc((void**)d_ghost_wg_block, ghost_wg_block.size() * sizeof(int));
+    if (st == hipSuccess)
+        st = hipMemcpy(*d_table, table.data(), table.size() * sizeof(gr::BatchBlock), hipMemcpyHostToDevice);
+    if (st == hipSuccess) st = hipMemcpy(*d_tile_block, tile_block.data(), ntiles * sizeof(int), hipMemcpyHostToDevice);
+    if (st == hipSuccess)
+        st = hipMemcpy(*d_ghost_wg_block, ghost_wg_block.data(), ghost_wg_block.size() * sizeof(int),
+                       hipMemcpyHostToDevice);
+    B->table = *d_table;
+    B->tile_block = *d_tile_block;
+    B->ghost_wg_block = *d_ghost_wg_block;
+    B->nblocks = (int)table.size();
+    B->one = table[0];
+    B->ntiles = ntiles;
+    B->ghost_wgs = (int)ghost_wg_block.size();
+    for (int d = 0; d < 3; ++d) B->dx[d] = table[0].b.dx[d];
+    return st;
+}
+
+extern "C" {
+
 int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** out) {
     int rc = use_device(e);
     if (rc) return rc;
@@ -330,51 +395,33 @@ int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** 
             return hipfail(e, st, "hipMalloc(level block)");
         }
     }
-    // one batched launch per stage needs one grid spacing for the level
-    for (int i = 1; i < nblocks; ++i)
-        for (int d = 0; d < 3; ++d)
-            if (L->blk[i].dx[d] != L->blk[0].dx[d]) {
-                gr_level_destroy(L);
-                return fail(e, GR_ERR_GEOM, "all blocks of a level must share dx");
+    L->src.assign(nblocks, gr::Sources{nullptr, nullptr, nullptr, nullptr});
+    // the whole level (ghost fills), then one group per distinct dx (stage launches)
+    std::vector<int> all(nblocks);
+    for (int i = 0; i < nblocks; ++i) all[i] = i;
+    hipError_t st = build_batch(L, all, L->table, &L->d_table, &L->d_tile_block, &L->d_ghost_wg_block, &L->batch);
+    for (int i = 0; st == hipSuccess && i < nblocks; ++i) {
+        bool placed = false;
+        for (gr_group& G : L->groups)
+            if (std::memcmp(L->blk[G.blocks[0]].dx, L->blk[i].dx, sizeof(L->blk[i].dx)) == 0) {
+                G.blocks.push_back(i);
+                placed = true;
+                break;
             }
-    L->table.resize(nblocks);
-    for (int i = 0; i < nblocks; ++i) {
-        L->table[i].b = L->blk[i];
-        L->table[i].buf[0] = L->u[i];
-        L->table[i].buf[1] = L->s1[i];
-        L->table[i].buf[2] = L->s2[i];
+        if (!placed) {
+            L->groups.emplace_back();
+            L->groups.back().blocks.push_back(i);
+        }
     }
-    std::vector<int> tile_block;
-    std::vector<int> ghost_wg_block;
-    const int ntiles = gr::batch_layout(L->table, tile_block, ghost_wg_block);
-    hipError_t st = hipMalloc((void**)&L->ham, maxstride * sizeof(double));
+    for (gr_group& G : L->groups) {
+        if (st != hipSuccess) break;
+        st = build_batch(L, G.blocks, G.table, &G.d_table, &G.d_tile_block, &G.d_ghost_wg_block, &G.batch);
+        if (st == hipSuccess) st = hipMalloc((void**)&G.flags, G.batch.ntiles);
+    }
     L->maxstride = maxstride;
+    if (st == hipSuccess) st = hipMalloc((void**)&L->ham, maxstride * sizeof(double));
     if (st == hipSuccess) st = hipMalloc((void**)&L->partial, L->partial_cap * sizeof(double));
     if (st == hipSuccess) st = hipMalloc((void**)&L->red, sizeof(double));
-    if (st == hipSuccess) st = hipMalloc((void**)&L->flags, ntiles);
-    if (st == hipSuccess) st = hipMalloc((void**)&L->d_table, nblocks * sizeof(gr::BatchBlock));
-    if (st == hipSuccess) st = hipMalloc((void**)&L->d_tile_block, ntiles * sizeof(int));
-    if (st == hipSuccess) st = hipMalloc((void**)&L->d_ghost_wg_block, ghost_wg_block.size() * sizeof(int));
-    if (st == hipSuccess)
-        st = hipMemcpy(L->d_table, L->table.data(), nblocks * sizeof(gr::BatchBlock), hipMemcpyHostToDevice);
-    if (st == hipSuccess)
-        st = hipMemcpy(L->d_tile_block, tile_block.data(), ntiles * sizeof(int), hipMemcpyHostToDevice);
-    if (st == hipSuccess)
-        st = hipMemcpy(L->d_ghost_wg_block, ghost_wg_block.data(), ghost_wg_block.size() * sizeof(int),
-                       hipMemcpyHostToDevice);
-    if (nblocks == 1) {
-        L->batch = gr::single_batch(L->blk[0], L->u[0], L->s1[0], L->s2[0]);
-    } else {
-        gr::Batch& B = L->batch;
-        B.table = L->d_table;
-        B.tile_block = L->d_tile_block;
-        B.ghost_wg_block = L->d_ghost_wg_block;
-        B.nblocks = nblocks;
-        B.one = L->table[0];
-        B.ntiles = ntiles;
-        B.ghost_wgs = (int)ghost_wg_block.size();
-        for (int d = 0; d < 3; ++d) B.dx[d] = L->blk[0].dx[d];
-    }
     if (st != hipSuccess) {
         gr_level_destroy(L);
         return hipfail(e, st, "hipMalloc(level scratch)");
@@ -400,7 +447,12 @@ int gr_level_destroy(gr_level* L) {
         if (p) (void)hipFree(p);
     for (double* p : L->s2)
         if (p) (void)hipFree(p);
-    if (L->flags) (void)hipFree(L->flags);
+    for (gr_group& G : L->groups) {
+        if (G.flags) (void)hipFree(G.flags);
+        if (G.d_table) (void)hipFree(G.d_table);
+        if (G.d_tile_block) (void)hipFree(G.d_tile_block);
+        if (G.d_ghost_wg_block) (void)hipFree(G.d_ghost_wg_block);
+    }
     if (L->d_table) (void)hipFree(L->d_table);
     if (L->d_tile_block) (void)hipFree(L->d_tile_block);
     if (L->d_ghost_wg_block) (void)hipFree(L->d_ghost_wg_block);
@@ -510,40 +562,57 @@ static hipError_t timed(gr_level* L, int kind, F&& launch) {
     return hipSuccess;
 }
 
-// one stage over every block of a level (or over the single-block batch B) in the
-// requested precision; sel = {input, u^n, output} buffer indices
-static hipError_t stage(gr_level* L, const gr::Batch& B, const int sel[3], const gr_ccz4_params* prm, int mode,
-                        const gr::StageCoef& c, int precision) {
-    const gr::Sources none{nullptr, nullptr, nullptr, nullptr};
+// one stage over the blocks of batch B in the requested precision; sel = {input, u^n,
+// output} buffer indices; flags: the mixed form's per-tile bytes (B.ntiles of them)
+static hipError_t stage_batch(gr_level* L, const gr::Batch& B, unsigned char* flags, const int sel[3],
+                              const gr_ccz4_params* prm, int mode, const gr::StageCoef& c, int precision) {
     const gr::CellParams P = cell_params(prm);
     if (precision == GR_PREC_MIXED)
-        return gr::launch_stage_mixed(B, sel, none, P, prm->ko_sigma, mode, c, L->eng->stream, L->flags,
+        return gr::launch_stage_mixed(B, sel, L->has_src, P, prm->ko_sigma, mode, c, L->eng->stream, flags,
                                       L->chi_fp64_below, L->eng->aux.stream ? &L->eng->aux : nullptr);
-    return gr::launch_stage_fp64(B, sel, none, P, prm->ko_sigma, mode, c, L->eng->stream);
+    return gr::launch_stage_fp64(B, sel, L->has_src, P, prm->ko_sigma, mode, c, L->eng->stream);
 }
 
-static hipError_t enqueue_rk3(gr_level* L, const gr_ccz4_params* prm, double dt, int bc, int pre, int precision) {
+// one stage over every block of a level: one launch per group of equal spacing
+static hipError_t stage(gr_level* L, const int sel[3], const gr_ccz4_params* prm, int mode, const gr::StageCoef& c,
+                        int precision) {
+    for (gr_group& G : L->groups) {
+        hipError_t st = stage_batch(L, G.batch, G.flags, sel, prm, mode, c, precision);
+        if (st != hipSuccess) return st;
+    }
+    return hipSuccess;
+}
+
+// stage weights of TimeIntegrator::sspRK3Step, src/main.cpp:673,679,685, and the buffers
+// each stage reads and writes (0 = u, 1 = s1, 2 = s2)
+static gr::StageCoef stage_coef(int k, double dt) {
+    if (k == 1) return gr::StageCoef{1.0, 0.0, 1.0 * dt};
+    if (k == 2) return gr::StageCoef{0.75, 0.25, 0.25 * dt};
+    return gr::StageCoef{1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt};
+}
+static const int kStageSel[4][3] = {{0, 0, 0}, {0, 0, 1}, {1, 0, 2}, {2, 0, 0}};
+
+// RK3 stage k (1..3) of every block: the stage kernel, then the ghost fill `bc` of the
+// buffer it wrote; pre (>= 0) fills the state's ghosts first
+static hipError_t enqueue_stage(gr_level* L, const gr_ccz4_params* prm, double dt, int k, int bc, int pre,
+                                int precision) {
     hipStream_t s = L->eng->stream;
     const gr::Batch& B = L->batch;
-    // stage weights of TimeIntegrator::sspRK3Step, src/main.cpp:673,679,685
-    const gr::StageCoef c1{1.0, 0.0, 1.0 * dt};
-    const gr::StageCoef c2{0.75, 0.25, 0.25 * dt};
-    const gr::StageCoef c3{1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt};
-    // buffers: 0 = u, 1 = s1, 2 = s2
-    static const int sel1[3] = {0, 0, 1}, sel2[3] = {1, 0, 2}, sel3[3] = {2, 0, 0};
     hipError_t st;
     if (pre >= 0 && (st = timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, 0, pre, s); })) != hipSuccess)
         return st;
-    if ((st = timed(L, 0, [&] { return stage(L, B, sel1, prm, gr::MODE_STAGE, c1, precision); })) != hipSuccess)
+    const int mode = k == 3 ? gr::MODE_FINAL : gr::MODE_STAGE;
+    if ((st = timed(L, k == 3 ? 1 : 0,
+                    [&] { return stage(L, kStageSel[k], prm, mode, stage_coef(k, dt), precision); })) != hipSuccess)
         return st;
-    if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, 1, bc, s); })) != hipSuccess) return st;
-    if ((st = timed(L, 0, [&] { return stage(L, B, sel2, prm, gr::MODE_STAGE, c2, precision); })) != hipSuccess)
-        return st;
-    if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, 2, bc, s); })) != hipSuccess) return st;
-    if ((st = timed(L, 1, [&] { return stage(L, B, sel3, prm, gr::MODE_FINAL, c3, precision); })) != hipSuccess)
-        return st;
-    if ((st = timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, 0, bc, s); })) != hipSuccess) return st;
-    return hipSuccess;
+    return timed(L, 2, [&] { return gr::launch_fill_ghosts_batch(B, kStageSel[k][2], bc, s); });
+}
+
+static hipError_t enqueue_rk3(gr_level* L, const gr_ccz4_params* prm, double dt, int bc, int pre, int precision) {
+    hipError_t st = enqueue_stage(L, prm, dt, 1, bc, pre, precision);
+    if (st == hipSuccess) st = enqueue_stage(L, prm, dt, 2, bc, GR_BC_NONE, precision);
+    if (st == hipSuccess) st = enqueue_stage(L, prm, dt, 3, bc, GR_BC_NONE, precision);
+    return st;
 }
 
 extern "C" {
@@ -563,8 +632,9 @@ int gr_level_rk3_step(gr_level* L, const gr_ccz4_params* prm, double dt, int pre
         GR_HIP(e, enqueue_rk3(L, prm, dt, bc_mode, pre_fill, precision));
         return GR_OK;
     }
-    const double key[10] = {dt, prm->kappa1, prm->kappa2, prm->eta, prm->ko_sigma, prm->chi_blend_center,
-                            prm->chi_blend_width, (double)precision, (double)bc_mode, (double)pre_fill};
+    const double key[11] = {dt, prm->kappa1, prm->kappa2, prm->eta, prm->ko_sigma, prm->chi_blend_center,
+                            prm->chi_blend_width, (double)precision, (double)bc_mode, (double)pre_fill,
+                            (double)L->generation};
     if (!L->have_graph || std::memcmp(key, L->gkey, sizeof(key)) != 0) {
         if (L->exec) GR_HIP(e, hipGraphExecDestroy(L->exec));
         if (L->graph) GR_HIP(e, hipGraphDestroy(L->graph));
@@ -593,7 +663,53 @@ int gr_level_set_mixed_policy(gr_level* L, double chi_fp64_below) {
     if (!L) return GR_ERR_ARG;
     if (!std::isfinite(chi_fp64_below)) return fail(L->eng, GR_ERR_ARG, "threshold must be finite");
     L->chi_fp64_below = chi_fp64_below;
-    L->have_graph = false; // the captured step bakes the threshold in
+    ++L->generation; // captured steps bake the threshold in
+    return GR_OK;
+}
+
+int gr_level_rk3_stage(gr_level* L, const gr_ccz4_params* prm, double dt, int stage_index, int precision,
+                       int bc_mode, int pre_fill) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if ((rc = check_params(e, prm))) return rc;
+    if (!std::isfinite(dt)) return fail(e, GR_ERR_ARG, "dt must be finite");
+    if (stage_index < 1 || stage_index > 3) return fail(e, GR_ERR_ARG, "stage must be 1, 2 or 3");
+    if (precision != GR_PREC_FP64 && precision != GR_PREC_MIXED) return fail(e, GR_ERR_ARG, "unknown precision");
+    if (bc_mode < 0 || bc_mode > 2) return fail(e, GR_ERR_ARG, "bc_mode must be 0, 1 or 2");
+    if (pre_fill < -1 || pre_fill > 2) return fail(e, GR_ERR_ARG, "pre_fill must be -1, 0, 1 or 2");
+    GR_HIP(e, enqueue_stage(L, prm, dt, stage_index, bc_mode, pre_fill, precision));
+    return GR_OK;
+}
+
+int gr_level_set_sources(gr_level* L, int b, const double* d_rho, const double* d_Si, const double* d_Sij,
+                         const double* d_S) {
+    int rc = level_block(L, b);
+    if (rc) return rc;
+    gr_engine* e = L->eng;
+    const int nsrc = (d_rho != nullptr) + (d_Si != nullptr) + (d_Sij != nullptr) + (d_S != nullptr);
+    if (nsrc != 0 && nsrc != 4) return fail(e, GR_ERR_ARG, "matter sources must be all null or all non-null");
+    // the block tables are read by launches already queued: change them in stream order
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    L->src[b] = gr::Sources{d_rho, d_Si, d_Sij, d_S};
+    L->has_src = false;
+    for (const gr::Sources& S : L->src) L->has_src = L->has_src || S.rho != nullptr;
+    auto refresh = [&](std::vector<gr::BatchBlock>& table, const std::vector<int>& which, gr::BatchBlock* d_table,
+                       gr::Batch& B) -> hipError_t {
+        for (size_t n = 0; n < which.size(); ++n) table[n].src = L->src[which[n]];
+        if (which.size() == 1) {
+            B.one.src = table[0].src;
+            return hipSuccess;
+        }
+        B.one.src = table[0].src;
+        return hipMemcpy(d_table, table.data(), table.size() * sizeof(gr::BatchBlock), hipMemcpyHostToDevice);
+    };
+    std::vector<int> all(L->blk.size());
+    for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
+    GR_HIP(e, refresh(L->table, all, L->d_table, L->batch));
+    for (gr_group& G : L->groups) GR_HIP(e, refresh(G.table, G.blocks, G.d_table, G.batch));
+    ++L->generation; // captured steps hold the launch form and the single-block arguments
     return GR_OK;
 }
 
@@ -607,8 +723,12 @@ int gr_level_rhs(gr_level* L, int b, const gr_ccz4_params* prm, int precision, d
     const size_t n = 22 * (size_t)L->blk[b].stride;
     GR_HIP(e, hipMemsetAsync(L->s1[b], 0, n * sizeof(double), e->stream));
     static const int sel[3] = {0, 1, 2};
-    GR_HIP(e, stage(L, gr::single_batch(L->blk[b], L->u[b], L->u[b], L->s1[b]), sel, prm, gr::MODE_RHS,
-                    gr::StageCoef{0.0, 0.0, 0.0}, precision));
+    unsigned char* flags = nullptr;
+    for (gr_group& G : L->groups)
+        for (int i : G.blocks)
+            if (i == b) flags = G.flags;
+    GR_HIP(e, stage_batch(L, gr::single_batch(L->blk[b], L->u[b], L->u[b], L->s1[b], L->src[b]), flags, sel, prm,
+                          gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0}, precision));
     GR_HIP(e, hipMemcpyAsync(host_rhs, L->s1[b], n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     GR_HIP(e, hipStreamSynchronize(e->stream));
     return GR_OK;
@@ -702,6 +822,90 @@ int gr_level_nonfinite(gr_level* L, long long* count) {
     return GR_OK;
 }
 
+int gr_level_first_nonfinite(gr_level* L, int* found, int* block, int* var, int* i, int* j, int* k, double* value) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!found || !block || !var || !i || !j || !k || !value) return fail(e, GR_ERR_ARG, "null output");
+    *found = 0;
+    unsigned long long* part = reinterpret_cast<unsigned long long*>(L->partial);
+    unsigned long long* res = reinterpret_cast<unsigned long long*>(L->red);
+    for (size_t b = 0; b < L->blk.size(); ++b) {
+        const gr::BlockDesc& B = L->blk[b];
+        int nparts = 0;
+        GR_HIP(e, gr::launch_first_nonfinite(B, L->u[b], part, L->partial_cap, &nparts, e->stream));
+        GR_HIP(e, gr::launch_min_partials(part, nparts, res, e->stream));
+        unsigned long long q = 0;
+        GR_HIP(e, hipMemcpyAsync(&q, res, sizeof(q), hipMemcpyDeviceToHost, e->stream));
+        GR_HIP(e, hipStreamSynchronize(e->stream));
+        if (q == ~0ull) continue;
+        const long long ncell = (long long)B.n[0] * B.n[1] * B.n[2];
+        const long long t = (long long)q % ncell;
+        *found = 1;
+        *block = (int)b;
+        *var = (int)((long long)q / ncell);
+        *i = B.ng + (int)(t % B.n[0]);
+        *j = B.ng + (int)((t / B.n[0]) % B.n[1]);
+        *k = B.ng + (int)(t / ((long long)B.n[0] * B.n[1]));
+        const long off = *var * B.stride + *i + (long)B.nt[0] * (*j + (long)B.nt[1] * *k);
+        GR_HIP(e, hipMemcpyAsync(value, L->u[b] + off, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        GR_HIP(e, hipStreamSynchronize(e->stream));
+        return GR_OK;
+    }
+    return GR_OK;
+}
+
+int gr_level_init_gauge_wave(gr_level* L, double amplitude, double wavelength) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!std::isfinite(amplitude) || !std::isfinite(wavelength) || wavelength == 0.0)
+        return fail(e, GR_ERR_ARG, "gauge wave needs a finite amplitude and a finite non-zero wavelength");
+    for (size_t b = 0; b < L->blk.size(); ++b)
+        GR_HIP(e, gr::launch_init_gauge_wave(L->blk[b], L->u[b], amplitude, wavelength, e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    return GR_OK;
+}
+
+} // extern "C"
+
+// per-block CFL partials of a level reduced into out[0] (one device double), in stream order
+static hipError_t enqueue_cfl(gr_level* L, double dt, double* partial, int per_block, double* out) {
+    hipStream_t s = L->eng->stream;
+    int used = 0;
+    for (size_t b = 0; b < L->blk.size(); ++b) {
+        int nparts = 0;
+        hipError_t st = gr::launch_cfl_max(L->blk[b], L->u[b], dt, partial + used, per_block, &nparts, s);
+        if (st != hipSuccess) return st;
+        used += nparts;
+    }
+    return gr::launch_max_partials(partial, used, out, s);
+}
+
+constexpr int kCflPartsPerBlock = 256;
+
+extern "C" {
+
+int gr_level_cfl(gr_level* L, double dt, double* max_cfl) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!max_cfl || !std::isfinite(dt)) return fail(e, GR_ERR_ARG, "null output or non-finite dt");
+    double* part = nullptr;
+    GR_HIP(e, hipMalloc((void**)&part, (L->blk.size() * kCflPartsPerBlock + 1) * sizeof(double)));
+    hipError_t st = enqueue_cfl(L, dt, part, kCflPartsPerBlock, part + L->blk.size() * kCflPartsPerBlock);
+    if (st == hipSuccess)
+        st = hipMemcpyAsync(max_cfl, part + L->blk.size() * kCflPartsPerBlock, sizeof(double), hipMemcpyDeviceToHost,
+                            e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    (void)hipFree(part);
+    if (st != hipSuccess) return hipfail(e, st, "advective CFL reduction");
+    return GR_OK;
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------------------------------
@@ -718,7 +922,13 @@ struct gr_hierarchy {
     hipGraphExec_t exec = nullptr;
     bool have_graph = false;
     double gkey[8] = {0};
-    std::vector<double> policy; // each level's mixed-precision threshold at capture
+    std::vector<double> policy; // each level's mixed-precision threshold and generation at capture
+    // advective CFL guard after every level step (src/main.cpp:1155-1176): one device value
+    // per evolve call of a coarse step, in subcycle order, and the level of each
+    bool cfl_on = false;
+    double* cfl = nullptr;
+    double* cfl_part = nullptr;
+    std::vector<int> cfl_level;
     // per level l > 0: prolongation stencil tables of every (fine block f, parent block c)
     // pair, pair f * nparents + c at ptab_ofs[l][f * nparents + c] entries into ptab[l]
     std::vector<gr::ProlongAxis*> ptab;
@@ -784,16 +994,23 @@ hipError_t h_fill(gr_hierarchy* H, int l) {
 // subcycle(l), src/amr/amr.cpp:303-350, without the regrid of its step 6. Its step 1
 // (prolongation into level l > 0) is the first half of fillGhostZones(l), which follows
 // at once and reads the same parent data, so the prolongation runs once.
-hipError_t h_subcycle(gr_hierarchy* H, int l, double dt, const gr_ccz4_params* prm, int precision) {
+hipError_t h_subcycle(gr_hierarchy* H, int l, double dt, const gr_ccz4_params* prm, int precision, int& slot) {
     hipError_t st = h_fill(H, l);
     if (st == hipSuccess) st = enqueue_rk3(H->lv[l], prm, dt, GR_BC_SOMMERFELD, GR_BC_NONE, precision);
+    // the reference's evolve callback ends with the CFL guard over the level's blocks
+    if (st == hipSuccess && H->cfl_on) st = enqueue_cfl(H->lv[l], dt, H->cfl_part, kCflPartsPerBlock, H->cfl + slot++);
     if (st == hipSuccess) st = h_fill(H, l);
     if (st == hipSuccess && l + 1 < (int)H->lv.size()) {
         const double dt_fine = dt / 2.0;
-        for (int r = 0; r < 2 && st == hipSuccess; ++r) st = h_subcycle(H, l + 1, dt_fine, prm, precision);
+        for (int r = 0; r < 2 && st == hipSuccess; ++r) st = h_subcycle(H, l + 1, dt_fine, prm, precision, slot);
         if (st == hipSuccess) st = h_restrict(H, l + 1);
     }
     return st;
+}
+
+hipError_t h_step(gr_hierarchy* H, double dt0, const gr_ccz4_params* prm, int precision) {
+    int slot = 0;
+    return h_subcycle(H, 0, dt0, prm, precision, slot);
 }
 
 int check_hier_level(gr_hierarchy* H, int level, int lo) {
@@ -858,7 +1075,56 @@ int gr_hierarchy_destroy(gr_hierarchy* H) {
     if (H->graph) (void)hipGraphDestroy(H->graph);
     for (gr::ProlongAxis* p : H->ptab)
         if (p) (void)hipFree(p);
+    if (H->cfl) (void)hipFree(H->cfl);
+    if (H->cfl_part) (void)hipFree(H->cfl_part);
     delete H;
+    return GR_OK;
+}
+
+int gr_hierarchy_set_cfl_monitor(gr_hierarchy* H, int enable) {
+    if (!H) return GR_ERR_ARG;
+    gr_engine* e = H->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (enable && !H->cfl) {
+        // one slot per evolve call of a coarse step: level l steps 2^l times
+        H->cfl_level.clear();
+        size_t maxblocks = 0;
+        for (size_t l = 0; l < H->lv.size(); ++l) {
+            for (size_t r = 0; r < ((size_t)1 << l); ++r) H->cfl_level.push_back((int)l);
+            maxblocks = std::max(maxblocks, H->lv[l]->blk.size());
+        }
+        // subcycle order: l, then two passes of l + 1 (recursively)
+        std::vector<int> order;
+        struct Walk {
+            static void run(std::vector<int>& o, int l, int n) {
+                o.push_back(l);
+                if (l + 1 < n)
+                    for (int r = 0; r < 2; ++r) run(o, l + 1, n);
+            }
+        };
+        Walk::run(order, 0, (int)H->lv.size());
+        H->cfl_level = order;
+        GR_HIP(e, hipMalloc((void**)&H->cfl, order.size() * sizeof(double)));
+        GR_HIP(e, hipMalloc((void**)&H->cfl_part, maxblocks * kCflPartsPerBlock * sizeof(double)));
+    }
+    H->cfl_on = enable != 0;
+    return GR_OK;
+}
+
+int gr_hierarchy_cfl(gr_hierarchy* H, double* values, int* levels, int cap, int* n) {
+    if (!H) return GR_ERR_ARG;
+    gr_engine* e = H->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (!n || (cap > 0 && (!values || !levels))) return fail(e, GR_ERR_ARG, "null output");
+    if (!H->cfl_on || !H->cfl) return fail(e, GR_ERR_STATE, "the CFL monitor is off (gr_hierarchy_set_cfl_monitor)");
+    const int m = (int)H->cfl_level.size();
+    *n = m;
+    if (cap < m) return fail(e, GR_ERR_ARG, "output capacity below the number of level steps per coarse step");
+    GR_HIP(e, hipMemcpyAsync(values, H->cfl, m * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    GR_HIP(e, hipStreamSynchronize(e->stream));
+    for (int i = 0; i < m; ++i) levels[i] = H->cfl_level[i];
     return GR_OK;
 }
 
@@ -894,14 +1160,18 @@ int gr_hierarchy_step(gr_hierarchy* H, const gr_ccz4_params* prm, double dt0, in
     bool profiling = false;
     for (gr_level* L : H->lv) profiling = profiling || L->profile;
     if (!use_graph || profiling) {
-        GR_HIP(e, h_subcycle(H, 0, dt0, prm, precision));
+        GR_HIP(e, h_step(H, dt0, prm, precision));
         return GR_OK;
     }
     const double key[8] = {dt0, prm->kappa1, prm->kappa2, prm->eta, prm->ko_sigma, prm->chi_blend_center,
                            prm->chi_blend_width, (double)precision};
     // the mixed-precision threshold of every level is baked into the captured kernels
     std::vector<double> policy;
-    for (gr_level* L : H->lv) policy.push_back(L->chi_fp64_below);
+    for (gr_level* L : H->lv) {
+        policy.push_back(L->chi_fp64_below);
+        policy.push_back((double)L->generation);
+    }
+    policy.push_back(H->cfl_on ? 1.0 : 0.0);
     const bool fresh = H->have_graph && std::memcmp(key, H->gkey, sizeof(key)) == 0 && policy == H->policy;
     if (!fresh) {
         if (H->exec) GR_HIP(e, hipGraphExecDestroy(H->exec));
@@ -910,7 +1180,7 @@ int gr_hierarchy_step(gr_hierarchy* H, const gr_ccz4_params* prm, double dt0, in
         H->graph = nullptr;
         H->have_graph = false;
         GR_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t st = h_subcycle(H, 0, dt0, prm, precision);
+        hipError_t st = h_step(H, dt0, prm, precision);
         hipGraph_t g = nullptr;
         hipError_t st2 = hipStreamEndCapture(e->stream, &g);
         if (st != hipSuccess) {
@@ -1094,6 +1364,13 @@ int gr_checkpoint_block(gr_checkpoint* c, int index, const double** data, long* 
     if (!c) return GR_ERR_ARG;
     if (!data || !stride) return fail(c->eng, GR_ERR_ARG, "null output");
     if (index < 0 || index >= (int)c->blocks.size()) return fail(c->eng, GR_ERR_ARG, "block index out of range");
+    if (c->pending) {
+        const hipError_t st = hipEventQuery(c->landed);
+        if (st == hipErrorNotReady)
+            return fail(c->eng, GR_ERR_STATE, "checkpoint transfer pending (gr_checkpoint_wait or _query first)");
+        if (st != hipSuccess) return hipfail(c->eng, st, "hipEventQuery");
+        c->pending = false;
+    }
     *data = c->host + c->ofs[index];
     *stride = c->blocks[index].first->blk[c->blocks[index].second].stride;
     return GR_OK;

@@ -445,7 +445,6 @@ struct TileIO32 {
 struct StageArgs {
     Batch batch;
     int sel_in, sel_un, sel_out;
-    Sources src;
     int ntiles;
     int xcd_remap;
     Stencil<double> K;
@@ -487,6 +486,7 @@ struct TileRef {
     const gdouble* in; // global address space: loads/stores wait on vmcnt only, never on LDS waits
     const gdouble* un;
     gdouble* out;
+    Sources src;
     int tix, tiy, tiz;
 };
 
@@ -497,6 +497,7 @@ __device__ __forceinline__ TileRef resolve_tile(const StageArgs& A, int t) {
     if (A.batch.table) {
         const BatchBlock& BB = A.batch.table[A.batch.tile_block[t]];
         r.b = BB.b;
+        r.src = BB.src;
         b0 = BB.buf[0];
         b1 = BB.buf[1];
         b2 = BB.buf[2];
@@ -505,6 +506,7 @@ __device__ __forceinline__ TileRef resolve_tile(const StageArgs& A, int t) {
         tiles1 = BB.tiles[1];
     } else {
         r.b = A.batch.one.b;
+        r.src = A.batch.one.src;
         b0 = A.batch.one.buf[0];
         b1 = A.batch.one.buf[1];
         b2 = A.batch.one.buf[2];
@@ -523,6 +525,25 @@ __device__ __forceinline__ TileRef resolve_tile(const StageArgs& A, int t) {
     r.tiy = (lt / tiles0) % tiles1;
     r.tiz = lt / (tiles0 * tiles1);
     return r;
+}
+
+// matter sources of the cell (zero for a block without sources: a vacuum block of a batch
+// that has sources elsewhere; zero sources give the vacuum RHS bit for bit)
+__device__ __forceinline__ void load_sources(const Sources& S, long cidx, CellSources& src) {
+    if (!S.rho) { // uniform over the workgroup
+        src.rho = src.S = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) src.Si[a] = 0.0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) src.Sij[a] = 0.0;
+        return;
+    }
+    src.rho = S.rho[cidx];
+    src.S = S.S[cidx];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) src.Si[a] = S.Si[3 * cidx + a];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) src.Sij[a] = S.Sij[6 * cidx + a];
 }
 
 // part `part` of a tile: TX x TY x TZ cells at x = (part % GR_FP64_SPLIT) * TX,
@@ -555,14 +576,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
     io.init(tid, b.ng + x0, b.ng + R.tiy * TY, b.ng + z0, b.nt[0], b.nt[1], b.nt[2]);
 
     CellSources src{};
-    if (HAS_SRC) {
-        src.rho = A.src.rho[cidx];
-        src.S = A.src.S[cidx];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) src.Si[a] = A.src.Si[3 * cidx + a];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) src.Sij[a] = A.src.Sij[6 * cidx + a];
-    }
+    if (HAS_SRC) load_sources(R.src, cidx, src);
     ccz4_cell<double, HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
 }
 
@@ -660,14 +674,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_m
     io.init(tid, b.ng + R.tix * TX, b.ng + R.tiy * TY, b.ng + R.tiz * TZ, b.nt[0], b.nt[1], b.nt[2]);
 
     CellSources src{};
-    if (HAS_SRC) {
-        src.rho = A.src.rho[cidx];
-        src.S = A.src.S[cidx];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) src.Si[a] = A.src.Si[3 * cidx + a];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) src.Sij[a] = A.src.Sij[6 * cidx + a];
-    }
+    if (HAS_SRC) load_sources(R.src, cidx, src);
     ccz4_cell<float, HAS_SRC, false>(io, io, A.K32, A.P, src);
 }
 
@@ -1308,6 +1315,121 @@ __global__ __launch_bounds__(RED_T) void sum_partials_kernel(const double* parti
     if (threadIdx.x == 0) out[0] = s;
 }
 
+// First non-finite interior value in the order of the reference's NaN scan
+// (src/main.cpp:1192-1214: field v, then k, j, i over the interior). Position q of field v
+// at interior cell (i', j', k') is v * ncell + (k' * n1 + j') * n0 + i'; each workgroup
+// takes the contiguous range [blockIdx.x * per, ...) of positions (consecutive threads on
+// consecutive x: coalesced) and writes the smallest non-finite position in it, or ~0.
+__global__ __launch_bounds__(RED_T) void first_nonfinite_kernel(BlockDesc b, const double* u,
+                                                                unsigned long long* partial, long per) {
+    __shared__ unsigned long long sh[RED_T];
+    const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
+    const long total = NV * ncell;
+    const long beg = (long)blockIdx.x * per;
+    const long end = beg + per < total ? beg + per : total;
+    unsigned long long best = ~0ull;
+    for (long q = beg + threadIdx.x; q < end; q += RED_T) {
+        const int v = (int)(q / ncell);
+        const long t = q - v * ncell;
+        const int i = b.ng + (int)(t % b.n[0]);
+        const int j = b.ng + (int)((t / b.n[0]) % b.n[1]);
+        const int k = b.ng + (int)(t / ((long)b.n[0] * b.n[1]));
+        const double x = u[v * b.stride + i + (long)b.nt[0] * (j + (long)b.nt[1] * k)];
+        if (!__builtin_isfinite(x)) {
+            best = (unsigned long long)q; // the thread's positions rise: the first is its smallest
+            break;
+        }
+    }
+    sh[threadIdx.x] = best;
+    __syncthreads();
+    for (int o = RED_T / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && sh[threadIdx.x + o] < sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(RED_T) void min_partials_kernel(const unsigned long long* partial, int n,
+                                                             unsigned long long* out) {
+    __shared__ unsigned long long sh[RED_T];
+    unsigned long long m = ~0ull;
+    for (int t = threadIdx.x; t < n; t += RED_T) m = partial[t] < m ? partial[t] : m;
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = RED_T / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && sh[threadIdx.x + o] < sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// the reference's max_adv_cfl = std::max(max_adv_cfl, local) keeps the running value when
+// local is NaN (the comparison is false), so NaN never enters the maximum
+__device__ __forceinline__ double cfl_max2(double a, double b) { return (a < b) ? b : a; }
+
+// per-substep advective CFL guard (src/main.cpp:1155-1176): over the interior,
+// |bx| * dt / dx0 + |by| * dt / dx1 + |bz| * dt / dx2, maximum per workgroup
+__global__ __launch_bounds__(RED_T) void cfl_max_kernel(BlockDesc b, const double* u, double dt, double* partial,
+                                                        long per) {
+#pragma clang fp contract(off)
+    __shared__ double sh[RED_T];
+    const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
+    const long beg = (long)blockIdx.x * per;
+    const long end = beg + per < ncell ? beg + per : ncell;
+    double m = 0.0;
+    for (long t = beg + threadIdx.x; t < end; t += RED_T) {
+        const int i = b.ng + (int)(t % b.n[0]);
+        const int j = b.ng + (int)((t / b.n[0]) % b.n[1]);
+        const int k = b.ng + (int)(t / ((long)b.n[0] * b.n[1]));
+        const long c = i + (long)b.nt[0] * (j + (long)b.nt[1] * k);
+        const double bx = fabs(u[BX * b.stride + c]), by = fabs(u[(BX + 1) * b.stride + c]),
+                     bz = fabs(u[(BX + 2) * b.stride + c]);
+        const double local = bx * dt / b.dx[0] + by * dt / b.dx[1] + bz * dt / b.dx[2];
+        m = cfl_max2(m, local);
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = RED_T / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = cfl_max2(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(RED_T) void max_partials_kernel(const double* partial, int n, double* out) {
+    __shared__ double sh[RED_T];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < n; t += RED_T) m = cfl_max2(m, partial[t]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = RED_T / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = cfl_max2(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// setGaugeWaveData (src/spacetime/ccz4.cpp:1190-1221): H = A sin(2 pi x / L) (the product
+// 2 pi x formed left to right, then divided by L), chi = (1 + H)^(-1/3), gt_xx = (1 + H) chi,
+// gt_yy = gt_zz = chi, alpha = sqrt(1 + H); the fields it does not set are 0, as on a
+// freshly constructed GridBlock
+__global__ void init_gauge_wave_kernel(BlockDesc b, double* u, double A, double L) {
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.stride) return;
+    const int i = (int)(t % b.nt[0]);
+    const double x = coord(b, 0, i);
+    const double H = A * sin(2.0 * PI * x / L);
+    const double detg = 1.0 + H;
+    const double chi = pow(detg, -1.0 / 3.0);
+    for (int v = 0; v < NV; ++v) u[v * b.stride + t] = 0.0;
+    u[CHI * b.stride + t] = chi;
+    u[(GXX + 0) * b.stride + t] = (1.0 + H) * chi;
+    u[(GXX + 3) * b.stride + t] = chi;
+    u[(GXX + 5) * b.stride + t] = chi;
+    u[ALPHA * b.stride + t] = sqrt(1.0 + H);
+}
+
 inline unsigned grid_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 } // namespace
@@ -1343,14 +1465,13 @@ void stage_tile_mixed(int* t3) {
 #define GR_XCD_REMAP 1
 #endif
 
-static StageArgs stage_args(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
-                            double ko_sigma, const StageCoef& cf, const unsigned char* flags, int want) {
+static StageArgs stage_args(const Batch& B, const int sel[3], const CellParams& P, double ko_sigma,
+                            const StageCoef& cf, const unsigned char* flags, int want) {
     StageArgs A;
     A.batch = B;
     A.sel_in = sel[0];
     A.sel_un = sel[1];
     A.sel_out = sel[2];
-    A.src = src;
     A.ntiles = B.ntiles;
     A.xcd_remap = GR_XCD_REMAP ? 1 : 0;
     BlockDesc g;
@@ -1365,7 +1486,7 @@ static StageArgs stage_args(const Batch& B, const int sel[3], const Sources& src
     }
     A.P = P;
     A.cf = cf;
-    A.ko_on = ko_sigma > 0.0 ? 1 : 0;
+    A.ko_on = ko_sigma <= 0.0 ? 0 : 1; // applyDissipation returns early only for sigma <= 0 (NaN dissipates)
     A.flags = flags;
     A.want = want;
     A.scan = 0;
@@ -1399,7 +1520,7 @@ int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, 
     return t;
 }
 
-Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out) {
+Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src) {
     Batch B;
     B.table = nullptr;
     B.tile_block = nullptr;
@@ -1409,6 +1530,7 @@ Batch single_batch(const BlockDesc& b, const double* in, const double* un, doubl
     B.one.buf[0] = const_cast<double*>(in);
     B.one.buf[1] = const_cast<double*>(un);
     B.one.buf[2] = out;
+    B.one.src = src;
     B.one.tile0 = 0;
     B.one.tiles[0] = (b.n[0] + TILE_X - 1) / TILE_X;
     B.one.tiles[1] = (b.n[1] + TILE_Y - 1) / TILE_Y;
@@ -1420,10 +1542,10 @@ Batch single_batch(const BlockDesc& b, const double* in, const double* un, doubl
     return B;
 }
 
-hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+hipError_t launch_stage_fp64(const Batch& B, const int sel[3], bool has_src, const CellParams& P,
                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags) {
-    StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 1);
+    StageArgs A = stage_args(B, sel, P, ko_sigma, cf, flags, 1);
     // with flags (the FP64 tiles of the mixed form: few, near the puncture) a small grid
     // gathers them instead of launching one workgroup per tile of the level
     int nblk = A.ntiles * FP64_PARTS;
@@ -1431,7 +1553,6 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& sr
         A.scan = 1;
         nblk = A.ntiles < FP64_SCAN_GRID ? A.ntiles : FP64_SCAN_GRID;
     }
-    const bool has_src = src.rho != nullptr;
     if (has_src) {
         if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, true>(A, nblk, s);
         if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, true>(A, nblk, s);
@@ -1449,10 +1570,10 @@ static hipError_t launch_mixed_t(const StageArgs& A, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+hipError_t launch_stage_mixed(const Batch& B, const int sel[3], bool has_src, const CellParams& P,
                               double ko_sigma, int mode, const StageCoef& cf, hipStream_t s, unsigned char* flags,
                               double chi_fp64_below, const AuxStream* aux) {
-    const StageArgs A = stage_args(B, sel, src, P, ko_sigma, cf, flags, 0);
+    const StageArgs A = stage_args(B, sel, P, ko_sigma, cf, flags, 0);
     classify_tiles_kernel<TILE_X, TILE_Y, TILE_Z>
         <<<dim3(A.ntiles), dim3(TILE_X * TILE_Y * TILE_Z), 0, s>>>(A, chi_fp64_below, flags);
     hipError_t st = hipGetLastError();
@@ -1462,11 +1583,11 @@ hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& s
     if (aux) {
         if ((st = hipEventRecord(aux->fork, s)) != hipSuccess) return st;
         if ((st = hipStreamWaitEvent(aux->stream, aux->fork, 0)) != hipSuccess) return st;
-        if ((st = launch_stage_fp64(B, sel, src, P, ko_sigma, mode, cf, aux->stream, flags)) != hipSuccess) return st;
+        if ((st = launch_stage_fp64(B, sel, has_src, P, ko_sigma, mode, cf, aux->stream, flags)) != hipSuccess)
+            return st;
         if ((st = hipEventRecord(aux->join, aux->stream)) != hipSuccess) return st;
     }
     // FP32 tiles (flag 0)
-    const bool has_src = src.rho != nullptr;
     if (has_src) {
         if (mode == MODE_RHS) st = launch_mixed_t<MODE_RHS, true>(A, s);
         else if (mode == MODE_STAGE) st = launch_mixed_t<MODE_STAGE, true>(A, s);
@@ -1479,7 +1600,7 @@ hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& s
     if (st != hipSuccess) return st;
     if (aux) return hipStreamWaitEvent(s, aux->join, 0);
     // FP64 tiles near the puncture (flag 1)
-    return launch_stage_fp64(B, sel, src, P, ko_sigma, mode, cf, s, flags);
+    return launch_stage_fp64(B, sel, has_src, P, ko_sigma, mode, cf, s, flags);
 }
 
 hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream_t s) {
@@ -1517,7 +1638,7 @@ hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc
 }
 
 hipError_t launch_dissipation(const BlockDesc& b, const double* u, double* rhs, double ko_sigma, hipStream_t s) {
-    if (!(ko_sigma > 0.0)) return hipSuccess;
+    if (ko_sigma <= 0.0) return hipSuccess;
     const long n = (long)b.n[0] * b.n[1] * b.n[2];
     dissipation_kernel<<<grid_for(n, 256), 256, 0, s>>>(b, u, rhs, make_stencil(b, ko_sigma));
     return hipGetLastError();
@@ -1560,6 +1681,38 @@ hipError_t launch_count_nonfinite(const BlockDesc& b, const double* u, double* p
                                   hipStream_t s) {
     const long per = reduce_split(b, cap, nparts);
     nonfinite_kernel<<<*nparts, RED_T, 0, s>>>(b, u, partial, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_first_nonfinite(const BlockDesc& b, const double* u, unsigned long long* partial, int cap,
+                                  int* nparts, hipStream_t s) {
+    const long total = NV * (long)b.n[0] * b.n[1] * b.n[2];
+    long per = (total + cap - 1) / cap;
+    if (per < RED_T) per = RED_T;
+    *nparts = (int)((total + per - 1) / per);
+    first_nonfinite_kernel<<<*nparts, RED_T, 0, s>>>(b, u, partial, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_min_partials(const unsigned long long* partial, int n, unsigned long long* out, hipStream_t s) {
+    min_partials_kernel<<<1, RED_T, 0, s>>>(partial, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_max(const BlockDesc& b, const double* u, double dt, double* partial, int cap, int* nparts,
+                          hipStream_t s) {
+    const long per = reduce_split(b, cap, nparts);
+    cfl_max_kernel<<<*nparts, RED_T, 0, s>>>(b, u, dt, partial, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_partials(const double* partial, int n, double* out, hipStream_t s) {
+    max_partials_kernel<<<1, RED_T, 0, s>>>(partial, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_gauge_wave(const BlockDesc& b, double* u, double A, double L, hipStream_t s) {
+    init_gauge_wave_kernel<<<grid_for(b.stride, 256), 256, 0, s>>>(b, u, A, L);
     return hipGetLastError();
 }
 

@@ -22,12 +22,22 @@ struct BlockDesc {
 
 enum StageMode : int { MODE_RHS = 0, MODE_STAGE = 1, MODE_FINAL = 2 };
 
+// Matter sources of one block (device arrays over all cells: rho[stride], S_i[3*stride]
+// and S_ij[6*stride] cell-major, S[stride]); all null = vacuum.
+struct Sources {
+    const double* rho;
+    const double* Si;
+    const double* Sij;
+    const double* S;
+};
+
 // One block of a batched launch: its geometry, its three state buffers (state u and the
-// two RK3 stage buffers, or for a single-block call: input, u^n, output) and the index of
-// its first stage tile in the launch.
+// two RK3 stage buffers, or for a single-block call: input, u^n, output), its matter
+// sources and the index of its first stage tile in the launch.
 struct BatchBlock {
     BlockDesc b;
     double* buf[3];
+    Sources src;
     int tile0;
     int tiles[3];
     int ghost_wg0; // first workgroup of the block in a batched ghost launch
@@ -56,8 +66,9 @@ struct Batch {
 // each block, the tile->block map and the ghost-workgroup->block map. Returns the total
 // number of tiles.
 int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, std::vector<int>& ghost_wg_block);
-// a one-block batch over (in, un, out)
-Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out);
+// a one-block batch over (in, un, out), with optional matter sources
+Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out,
+                   const Sources& src = Sources{nullptr, nullptr, nullptr, nullptr});
 
 struct StageCoef {
     double a1;   // weight of u^n (read pointwise from `un`)
@@ -65,21 +76,14 @@ struct StageCoef {
     double a3dt; // a3 * dt, weight of the RHS
 };
 
-struct Sources {
-    const double* rho;
-    const double* Si;
-    const double* Sij;
-    const double* S;
-};
-
 // stencil constants for a block, with the reference's operation order
 Stencil<double> make_stencil(const BlockDesc& b, double ko_sigma);
 
 // RHS / RK3 stage kernel (FP64) over every tile of a batch. sel = {input, u^n, output}
 // buffer indices; input ghosts must be filled; u^n is ignored for MODE_RHS; the output
-// receives the RHS (MODE_RHS) or the new stage state. Sources apply to single-block
-// batches only.
-hipError_t launch_stage_fp64(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+// receives the RHS (MODE_RHS) or the new stage state. has_src: some block of the batch
+// carries matter sources (blocks whose sources are null are vacuum).
+hipError_t launch_stage_fp64(const Batch& B, const int sel[3], bool has_src, const CellParams& P,
                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags = nullptr);
 
@@ -94,7 +98,7 @@ struct AuxStream {
 // the tile is below chi_fp64_below), then runs the FP32 kernel on the other tiles and the
 // FP64 kernel on those (on `aux` beside the FP32 kernel when given, else after it on s).
 // flags: device buffer of B.ntiles bytes.
-hipError_t launch_stage_mixed(const Batch& B, const int sel[3], const Sources& src, const CellParams& P,
+hipError_t launch_stage_mixed(const Batch& B, const int sel[3], bool has_src, const CellParams& P,
                               double ko_sigma, int mode, const StageCoef& cf, hipStream_t s, unsigned char* flags,
                               double chi_fp64_below, const AuxStream* aux = nullptr);
 // number of stage tiles of a block
@@ -145,6 +149,21 @@ hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStr
 // partial[blockIdx] = number of non-finite interior values over all 22 fields
 hipError_t launch_count_nonfinite(const BlockDesc& b, const double* u, double* partial, int cap, int* nparts,
                                   hipStream_t s);
+// First non-finite interior value of a block in the reference's scan order (field, then k, j,
+// i): partial[blockIdx] = smallest position q = v * ncell + (k * n1 + j) * n0 + i (interior-
+// relative) of a non-finite value in the workgroup's range, or ~0 if none; then
+// launch_min_partials reduces them into out[0].
+hipError_t launch_first_nonfinite(const BlockDesc& b, const double* u, unsigned long long* partial, int cap,
+                                  int* nparts, hipStream_t s);
+hipError_t launch_min_partials(const unsigned long long* partial, int n, unsigned long long* out, hipStream_t s);
+// Advective CFL of the shift over the interior, sum_d |beta^d| * dt / dx_d, as the reference's
+// per-substep guard computes it (max over cells, NaN ignored): partial[blockIdx] = the
+// workgroup's maximum; launch_max_partials reduces n partials into out[0].
+hipError_t launch_cfl_max(const BlockDesc& b, const double* u, double dt, double* partial, int cap, int* nparts,
+                          hipStream_t s);
+hipError_t launch_max_partials(const double* partial, int n, double* out, hipStream_t s);
+// setGaugeWaveData (src/spacetime/ccz4.cpp:1190-1221) on every cell; other fields set to 0
+hipError_t launch_init_gauge_wave(const BlockDesc& b, double* u, double A, double L, hipStream_t s);
 
 // tile shape of the FP64 stage kernel
 void stage_tile_fp64(int* t3);

@@ -1,11 +1,14 @@
 // The subset of YAML that GRANITE's parameter files use (benchmarks/*/params.yaml): nested
 // block mappings by indentation, scalars, flow sequences [a, b, c], block sequences of
-// mappings ("- key: value" items), and # comments. It stands in for yaml-cpp, which the
+// mappings ("- key: value" items), folded (>) and literal (|) block scalars with their
+// chomping indicators (the descriptions of benchmarks/B5_star/params.yaml:10 and
+// benchmarks/validation_tests.yaml), and # comments. It stands in for yaml-cpp, which the
 // reference driver links (src/main.cpp:730-830) and which is not available here.
 //
 // Header-only C++17. Errors throw std::runtime_error with the line number.
 #pragma once
 
+#include <algorithm>
 #include <cstdlib>
 #include <fstream>
 #include <memory>
@@ -59,6 +62,8 @@ struct Line {
     int indent;
     std::string text;
     int number;
+    bool block = false;     // "key: >" or "key: |": the value is block_text
+    std::string block_text;
 };
 
 inline std::string trim(const std::string& s) {
@@ -76,6 +81,8 @@ inline std::string strip_comment(const std::string& s) {
     }
     return s;
 }
+
+inline bool is_item_text(const std::string& t) { return t == "-" || t.rfind("- ", 0) == 0; }
 
 inline YNode scalar(std::string t) {
     t = trim(t);
@@ -157,7 +164,12 @@ private:
             const std::string key = trim(l.text.substr(0, c));
             const std::string rest = l.text.substr(c + 1);
             ++i;
-            if (!trim(rest).empty()) {
+            if (l.block) {
+                YNode v;
+                v.kind = YNode::Scalar;
+                v.text = l.block_text;
+                n.map.emplace_back(key, v);
+            } else if (!trim(rest).empty()) {
                 n.map.emplace_back(key, value(rest, l.number));
             } else if (i < L_.size() && (L_[i].indent > indent || (L_[i].indent == indent && is_item(L_[i])))) {
                 n.map.emplace_back(key, block(i, L_[i].indent));
@@ -171,19 +183,89 @@ private:
 
 } // namespace yaml_detail
 
+namespace yaml_detail {
+
+// "key: >", "key: |-" ...: the block-scalar header of a mapping line, or '\0'
+inline char block_header(const std::string& text, char* chomp) {
+    const size_t c = text.find(':');
+    if (c == std::string::npos) return '\0';
+    const std::string v = trim(text.substr(c + 1));
+    if (v.empty() || (v[0] != '>' && v[0] != '|')) return '\0';
+    *chomp = ' ';
+    for (size_t q = 1; q < v.size(); ++q) {
+        if (v[q] == '-' || v[q] == '+') *chomp = v[q];
+        else if (!(v[q] >= '1' && v[q] <= '9')) return '\0';
+    }
+    return v[0];
+}
+
+// The body of a block scalar: the lines after the header that are blank or indented deeper
+// than the key, with the indentation of its first line removed. Literal (|) keeps the line
+// breaks, folded (>) joins lines with spaces (each blank line becomes one line break); chomping
+// '-' strips the final line break, '+' keeps trailing blank lines, clip keeps one.
+inline std::string block_body(const std::vector<std::string>& raw, size_t& r, int key_indent, char style, char chomp) {
+    std::vector<std::string> body;
+    int ind = -1;
+    while (r < raw.size()) {
+        const std::string& l = raw[r];
+        const bool blank = trim(l).empty();
+        int li = 0;
+        while (li < (int)l.size() && l[li] == ' ') ++li;
+        if (!blank && li <= key_indent) break;
+        if (!blank && ind < 0) ind = li;
+        body.push_back(blank ? std::string() : l.substr(std::min<size_t>((size_t)std::max(ind, 0), l.size())));
+        ++r;
+    }
+    size_t trailing = 0;
+    while (!body.empty() && body.back().empty()) {
+        body.pop_back();
+        ++trailing;
+    }
+    std::string out;
+    for (size_t q = 0; q < body.size(); ++q) {
+        if (style == '|') {
+            if (q > 0) out += "\n";
+        } else if (body[q].empty()) {
+            out += "\n"; // a blank line of a folded scalar is one line break
+        } else if (q > 0 && !body[q - 1].empty()) {
+            out += " ";
+        }
+        out += body[q];
+    }
+    if (chomp == '+') out += std::string(trailing + (body.empty() ? 0 : 1), '\n');
+    else if (chomp == ' ' && !body.empty()) out += "\n";
+    return out;
+}
+
+} // namespace yaml_detail
+
 inline YNode parse_yaml(const std::string& text) {
     std::vector<yaml_detail::Line> lines;
-    std::stringstream ss(text);
-    std::string raw;
-    int number = 0;
-    while (std::getline(ss, raw)) {
-        ++number;
-        const std::string s = yaml_detail::strip_comment(raw);
+    std::vector<std::string> raw;
+    {
+        std::stringstream ss(text);
+        std::string l;
+        while (std::getline(ss, l)) {
+            if (!l.empty() && l.back() == '\r') l.pop_back();
+            raw.push_back(l);
+        }
+    }
+    size_t r = 0;
+    while (r < raw.size()) {
+        const int number = (int)r + 1;
+        const std::string s = yaml_detail::strip_comment(raw[r++]);
         if (yaml_detail::trim(s).empty()) continue;
         int ind = 0;
         while (ind < (int)s.size() && s[ind] == ' ') ++ind;
         if (ind < (int)s.size() && s[ind] == '\t') throw std::runtime_error("yaml line " + std::to_string(number) + ": tab indentation");
-        lines.push_back({ind, yaml_detail::trim(s), number});
+        yaml_detail::Line line{ind, yaml_detail::trim(s), number};
+        char chomp = ' ';
+        const char style = yaml_detail::is_item_text(line.text) ? '\0' : yaml_detail::block_header(line.text, &chomp);
+        if (style) {
+            line.block = true;
+            line.block_text = yaml_detail::block_body(raw, r, ind, style, chomp);
+        }
+        lines.push_back(std::move(line));
     }
     return yaml_detail::Parser(std::move(lines)).parse();
 }

@@ -5,9 +5,12 @@
 //
 // Reads the same parameter file keys as the reference driver (grid, time, ccz4, io,
 // initial_data, black_holes; src/main.cpp:730-830) and prints the same telemetry lines
-// (banner, grid, dt, tracking spheres, AMR levels, "[NaN@step=...]" for the first 20 steps,
-// "step= t= α_center= ||H||₂= [Blocks: n]" every output_interval steps), which is what
-// scripts/run_granite.py and python/granite_analysis/core/parsers.py:24-49 consume.
+// (banner, grid, dt, tracking spheres, AMR levels, "[CFL-GUARD] Advection CFL=..." after a
+// level step whose advective CFL exceeds 0.95, "[NaN@step=N] ST var=v (i,j,k) = val" or
+// "[NaN@step=N] all finite" for the first 20 steps, "step= t= α_center= ||H||₂= [Blocks: n]"
+// every output_interval steps), which is what scripts/run_granite.py and
+// python/granite_analysis/core/parsers.py:24-49 consume. The NaN scan and the CFL guard run
+// on the device (gr_level_first_nonfinite, the hierarchy's CFL monitor).
 //
 // Differences from the reference driver, all outside the CCZ4 + RK3 hot path:
 //   * vacuum only: no GRMHD (matter sources are zero; the reference's atmosphere density is
@@ -27,7 +30,8 @@
 //       (var v = GridBlock::rawData(v), ghosts included). The state is staged through
 //       gr_checkpoint (device snapshot, then a copy stream to pinned memory), so the file is
 //       written while the evolution goes on;
-//   * no CFL-guard line.
+//   * the CFL-guard lines of a coarse step are printed once the step has run on the device
+//     (the reference prints each inside its level step; the order of the lines is the same).
 // --dry-run prints the geometry and exits without touching a GPU; --no-checkpoints skips
 // the checkpoint files.
 #include "granite_ccz4.h"
@@ -42,6 +46,7 @@
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -303,6 +308,9 @@ int main(int argc, char** argv) {
             lv.push_back(L);
         }
         ok(gr_hierarchy_create(eng, (int)lv.size(), lv.data(), &H));
+        ok(gr_hierarchy_set_cfl_monitor(H, 1));
+        std::vector<double> cfl(1u << lv.size());
+        std::vector<int> cfl_level(cfl.size());
         // initial data on level 0 (src/main.cpp:962-1071), finer levels prolongated from
         // their parents as regrid does (src/amr/amr.cpp:290-293), then fillGhostZones(0)
         if (punctures) {
@@ -344,16 +352,31 @@ int main(int argc, char** argv) {
         while (t < p.t_final && step < p.max_steps) {
             if (t + dt > p.t_final) dt = p.t_final - t;
             ok(gr_hierarchy_step(H, &prm, dt, prec, 1));
+            // the advective CFL guard of every level step of this coarse step
+            // (src/main.cpp:1155-1176), in subcycle order
+            int ncfl = 0;
+            ok(gr_hierarchy_cfl(H, cfl.data(), cfl_level.data(), (int)cfl.size(), &ncfl));
+            for (int q = 0; q < ncfl; ++q)
+                if (cfl[q] > 0.95) std::cout << "  [CFL-GUARD] Advection CFL=" << cfl[q] << " > 0.95 at sub-step!\n";
             t += dt;
             ++step;
+            // per-step NaN scan of the first 20 steps (src/main.cpp:1192-1214): the first
+            // non-finite interior value, blocks coarsest level first, then var, k, j, i
             if (step <= 20) {
-                long long bad = 0, n = 0;
+                bool found_nan = false;
                 for (gr_level* L : lv) {
-                    ok(gr_level_nonfinite(L, &n));
-                    bad += n;
+                    int found = 0, blk = 0, var = 0, ci = 0, cj = 0, ck = 0;
+                    double val = 0.0;
+                    ok(gr_level_first_nonfinite(L, &found, &blk, &var, &ci, &cj, &ck, &val));
+                    if (!found) continue;
+                    std::ostringstream msg;
+                    msg << "  [NaN@step=" << step << "] ST var=" << var << " (" << ci << "," << cj << "," << ck << ")"
+                        << " = " << val;
+                    std::cout << msg.str() << std::endl;
+                    found_nan = true;
+                    break;
                 }
-                if (bad == 0) std::cout << "  [NaN@step=" << step << "] all finite\n";
-                else std::cout << "  [NaN@step=" << step << "] ST non-finite values: " << bad << "\n";
+                if (!found_nan) std::cout << "  [NaN@step=" << step << "] all finite\n";
             }
             if (step % p.output_interval == 0) {
                 double h2 = 0.0;

@@ -114,7 +114,10 @@ int gr_ccz4_rhs_device(gr_engine* eng, const gr_geom* geom, const gr_ccz4_params
 
 /* A set of blocks evolved together (the blocks the reference passes to its per-level
  * callback EvolutionStepFunc, include/granite/amr/amr.hpp:43; body src/main.cpp:1142-1177).
- * Device memory per block: state u plus two stage buffers, 3 * 22 * stride doubles. */
+ * Device memory per block: state u plus two stage buffers, 3 * 22 * stride doubles.
+ * Blocks whose dx are bit-identical advance in one stage launch; blocks with another dx
+ * (separately built boxes of one level can differ by an ulp) in a launch of their own, each
+ * block with the stencil constants of its own dx, as the reference evaluates it. */
 int gr_level_create(gr_engine* eng, int nblocks, const gr_geom* geoms, gr_level** out);
 int gr_level_destroy(gr_level* lvl);
 int gr_level_upload(gr_level* lvl, int block, const double* host_u);
@@ -130,8 +133,23 @@ int gr_level_init_brill_lindquist(gr_level* lvl, int nbh, const double* mass, co
  * splitmix64 at counters 2q and 2q+1, then Box-Muller (cosine branch). */
 int gr_level_add_noise(gr_level* lvl, uint64_t seed, double amp, uint32_t field_mask);
 
+/* setGaugeWaveData (src/spacetime/ccz4.cpp:1190-1221) on every cell of every block:
+ * H = A sin(2 pi x / L), chi = (1+H)^(-1/3), gt_xx = (1+H) chi, gt_yy = gt_zz = chi,
+ * alpha = sqrt(1+H); all other fields 0. */
+int gr_level_init_gauge_wave(gr_level* lvl, double amplitude, double wavelength);
+
 /* Fill the ghost cells of every block's state with `mode`. */
 int gr_level_fill_ghosts(gr_level* lvl, int mode);
+
+/* Matter sources of block `block` for the stage kernels of this level (device memory of the
+ * engine's device, borrowed until replaced): rho[stride], Si[3*stride] (AoS), Sij[6*stride]
+ * (AoS, xx,xy,xz,yy,yz,zz), S[stride], the arrays grmhd.computeMatterSources fills and
+ * sspRK3Step hands to computeRHS (src/main.cpp:295-301). All four null = vacuum (the default).
+ * The arrays are read by every stage launched after this call; a caller that recomputes
+ * the sources from each stage's state, as the reference does, writes them between
+ * gr_level_rk3_stage calls. Waits for the level's queued work. */
+int gr_level_set_sources(gr_level* lvl, int block, const double* d_rho, const double* d_Si, const double* d_Sij,
+                         const double* d_S);
 
 /* One SSP-RK3 step of every block of the level, in the reference order of
  * TimeIntegrator::sspRK3Step (src/main.cpp:523-689): three (RHS + dissipation + stage
@@ -143,6 +161,14 @@ int gr_level_fill_ghosts(gr_level* lvl, int mode);
  * given (dt, params, modes)). Asynchronous: ordered on the engine stream. */
 int gr_level_rk3_step(gr_level* lvl, const gr_ccz4_params* prm, double dt, int precision, int bc_mode,
                       int pre_fill, int use_graph);
+
+/* One stage (stage_index 1, 2 or 3) of the step gr_level_rk3_step runs, with the same
+ * buffers, weights, floors, algebraic constraints (stage 3) and ghost fill `bc_mode` of the
+ * buffer it writes; pre_fill as there (stage 1 only, normally). The seam for callers that
+ * recompute the matter sources from each stage's state between stages
+ * (TimeIntegrator::sspRK3Step, src/main.cpp:243-304,673-689). Asynchronous, not captured. */
+int gr_level_rk3_stage(gr_level* lvl, const gr_ccz4_params* prm, double dt, int stage_index, int precision,
+                       int bc_mode, int pre_fill);
 
 /* Mixed-precision policy: tiles whose smallest chi is below chi_fp64_below (or not
  * finite) run the FP64 stage kernel; all other tiles the FP32 one. Default 0.2. */
@@ -164,6 +190,18 @@ int gr_level_constraints_l2(gr_level* lvl, double* sumsq, long long* count);
 /* Number of non-finite values in the interiors of all blocks' state (NaN scan of
  * src/main.cpp:1192-1214). */
 int gr_level_nonfinite(gr_level* lvl, long long* count);
+
+/* The first non-finite interior value in the reference's scan order (src/main.cpp:1192-1214:
+ * blocks in order, then field v, then k, j, i): *found = 1 and its block, field, cell
+ * (counted with ghosts, as the reference prints it) and value; *found = 0 when every interior
+ * value is finite. Synchronous. */
+int gr_level_first_nonfinite(gr_level* lvl, int* found, int* block, int* var, int* i, int* j, int* k,
+                             double* value);
+
+/* Advective CFL of the state, max over the interiors of all blocks of
+ * |beta^x| dt/dx + |beta^y| dt/dy + |beta^z| dt/dz (NaN cells ignored), the quantity of the
+ * reference's per-substep guard (src/main.cpp:1155-1176). Synchronous. */
+int gr_level_cfl(gr_level* lvl, double dt, double* max_cfl);
 
 /* One value of a block's device state: field var at cell (i, j, k) counted with ghosts, as
  * GridBlock::data(var, i, j, k) (include/granite/core/grid.hpp:107-114), e.g. the lapse at
@@ -212,6 +250,15 @@ int gr_hierarchy_step(gr_hierarchy* h, const gr_ccz4_params* prm, double dt0, in
  * sum pointer may be null. */
 int gr_hierarchy_constraints_l2(gr_hierarchy* h, double* ham_sumsq, double* mom_sumsq, long long* count);
 
+/* enable != 0: every coarse step also evaluates, after each level's RK3 step, the advective
+ * CFL guard of the reference's evolve callback (src/main.cpp:1155-1176) on the device (inside
+ * the captured graph). gr_hierarchy_cfl returns the values of the last coarse step, one per
+ * level step in subcycle order, with each one's level; *n = the number of level steps per
+ * coarse step (sum of 2^l). The reference prints "[CFL-GUARD] Advection CFL=v > 0.95 at
+ * sub-step!" for each value above 0.95. */
+int gr_hierarchy_set_cfl_monitor(gr_hierarchy* h, int enable);
+int gr_hierarchy_cfl(gr_hierarchy* h, double* values, int* levels, int cap, int* n);
+
 /* ---- checkpoint staging ---------------------------------------------------------------------
  * The data HDF5Writer::writeCheckpoint (src/io/hdf5_io.cpp:166-232) writes at
  * step % checkpoint_interval == 0 or t >= t_final (src/main.cpp:1256-1261): every block of the
@@ -236,7 +283,8 @@ int gr_checkpoint_query(gr_checkpoint* c, int* done);
 /* Blocks until the last begin's transfer has completed. */
 int gr_checkpoint_wait(gr_checkpoint* c);
 /* Number of blocks, and the host view of block `index` (valid after gr_checkpoint_wait until
- * the next begin): var v is data[v * stride .. v * stride + stride), GridBlock::rawData(v). */
+ * the next begin): var v is data[v * stride .. v * stride + stride), GridBlock::rawData(v).
+ * Fails with GR_ERR_STATE while the last begin's transfer is still running. */
 int gr_checkpoint_nblocks(gr_checkpoint* c, int* nblocks);
 int gr_checkpoint_block(gr_checkpoint* c, int index, const double** data, long* stride);
 

@@ -535,7 +535,7 @@ void rhs_block(const Geom& g, const Params& P, const double* u, double* rhs, con
         const long c = g.at(i, j, k);
         for (int v = 0; v < NV; ++v) rhs[v * g.stride + c] = o[v];
     });
-    if (P.ko_sigma > 0.0) {
+    if (!(P.ko_sigma <= 0.0)) { // a NaN sigma dissipates (src/spacetime/ccz4.cpp:873-874)
         for_interior(g, [&](int i, int j, int k) {
             double dd[NV];
             cell_ko(g, P.ko_sigma, u, i, j, k, dd);
@@ -786,23 +786,33 @@ void orc_combine(const int* n, int ng, const double* lo, const double* dx, doubl
 // stage and rhs are caller scratch of the same shape as u. bc_mode as orc_fill_ghosts.
 // Pinned by: the pieces above, plus the exact SSP-RK3 amplification on a linear ODE
 // (flat space with beta = Gamma-hat = 0 reduces the shift equation to d beta/dt = -eta beta).
-void orc_rk3_step(const int* n, int ng, const double* lo, const double* dx, const double* prm, double* u,
-                  double* stage, double* rhs, double dt, int bc_mode) {
+// One SSP-RK3 step with fixed matter sources in every stage (all null = vacuum): the
+// stage sequence of TimeIntegrator::sspRK3Step (src/main.cpp:673-689) with computeRHS
+// receiving the same rho, S_i, S_ij, S each stage (src/main.cpp:295-301, where the reference
+// recomputes them from the hydro state, which this restatement holds fixed).
+void orc_rk3_step_src(const int* n, int ng, const double* lo, const double* dx, const double* prm, double* u,
+                      double* stage, double* rhs, double dt, int bc_mode, const double* rho, const double* Si,
+                      const double* Sij, const double* S) {
     Geom g = make_geom(n, ng, lo, dx);
     Params P{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5]};
-    rhs_block(g, P, u, rhs, nullptr, nullptr, nullptr, nullptr);
+    rhs_block(g, P, u, rhs, rho, Si, Sij, S);
     combine(g, stage, 1.0, u, 0.0, u, 1.0, dt, rhs);
     apply_floors(g, stage);
     fill_bc(g, stage, bc_mode);
-    rhs_block(g, P, stage, rhs, nullptr, nullptr, nullptr, nullptr);
+    rhs_block(g, P, stage, rhs, rho, Si, Sij, S);
     combine(g, stage, 0.75, u, 0.25, stage, 0.25, dt, rhs);
     apply_floors(g, stage);
     fill_bc(g, stage, bc_mode);
-    rhs_block(g, P, stage, rhs, nullptr, nullptr, nullptr, nullptr);
+    rhs_block(g, P, stage, rhs, rho, Si, Sij, S);
     combine(g, u, 1.0 / 3.0, u, 2.0 / 3.0, stage, 2.0 / 3.0, dt, rhs);
     apply_floors(g, u);
     apply_algebraic(g, u);
     fill_bc(g, u, bc_mode);
+}
+
+void orc_rk3_step(const int* n, int ng, const double* lo, const double* dx, const double* prm, double* u,
+                  double* stage, double* rhs, double dt, int bc_mode) {
+    orc_rk3_step_src(n, ng, lo, dx, prm, u, stage, rhs, dt, bc_mode, nullptr, nullptr, nullptr, nullptr);
 }
 
 // Initial data ---------------------------------------------------------------

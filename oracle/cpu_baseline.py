@@ -32,7 +32,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--half-width", type=float, default=16.0)
-    ap.add_argument("--reps", type=int, default=3, help="RHS evaluations timed (3 = one RK3 step's RHS work)")
+    ap.add_argument("--steps", type=int, default=5, help="RK3 steps' worth of RHS work per timed run")
+    ap.add_argument("--runs", type=int, default=3, help="timed runs; the median is reported")
     args = ap.parse_args()
     n = args.n
     w = args.half_width
@@ -40,32 +41,37 @@ def main():
     port = Oracle(fast=True)
     u = port.init_bl(g, [1.0], [[0.0, 0.0, 0.0]])
     threads = port.num_threads()
+    times = []
     if Reference.available(fast=True):
         ref = Reference(fast=True)
-        gw = Geom((32, 32, 32), 4, (-4.0,) * 3, (4.0,) * 3)
-        ref.rhs(gw, port.init_bl(gw, [1.0], [[0.0, 0.0, 0.0]]))  # warm-up (threads, pages)
         rhs = g.zeros()
-        t0 = time.perf_counter()
-        for _ in range(args.reps):
-            ref.rhs(g, u, rhs)
-        dt = (time.perf_counter() - t0) * 3 / args.reps
+        ref.rhs(g, u, rhs)  # warm-up (threads, pages)
+        for _ in range(args.runs):
+            t0 = time.perf_counter()
+            for _ in range(3 * args.steps):
+                ref.rhs(g, u, rhs)
+            times.append((time.perf_counter() - t0) / args.steps)
         kind = "reference"
-        scaled = "" if args.reps == 3 else ", time scaled to 3"
-        sample = (f"{args.reps} x computeRHS+applyDissipation of the compiled reference on the {n}^3 Brill-Lindquist "
-                  f"block{scaled} (the RHS work of one RK3 step; combine/floor/ghost passes excluded)")
+        sample = (f"median of {args.runs} runs, each {args.steps} RK3 steps' RHS work (3 x computeRHS + "
+                  f"applyDissipation per step) of the compiled reference on the {n}^3 Brill-Lindquist block, after "
+                  f"one warm-up evaluation; the combine / floor / ghost passes of a step are excluded")
     else:
         stage = g.zeros()
         rhs = g.zeros()
-        port.rk3_step(Geom((32, 32, 32), 4, (-4.0,) * 3, (4.0,) * 3),
-                      port.init_bl(Geom((32, 32, 32), 4, (-4.0,) * 3, (4.0,) * 3), [1.0], [[0, 0, 0]]), 0.0625)
-        t0 = time.perf_counter()
-        port.rk3_step(g, u, 0.0625, 0, stage=stage, rhs=rhs)
-        dt = time.perf_counter() - t0
+        port.rk3_step(g, u.copy(), 0.0625, 0, stage=stage, rhs=rhs)  # warm-up
+        for _ in range(args.runs):
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                port.rk3_step(g, u, 0.0625, 0, stage=stage, rhs=rhs)
+            times.append((time.perf_counter() - t0) / args.steps)
         kind = "port"
-        sample = f"one full SSP-RK3 step of the oracle on the {n}^3 Brill-Lindquist block"
+        sample = (f"median of {args.runs} runs of {args.steps} full SSP-RK3 steps of the oracle on the {n}^3 "
+                  f"Brill-Lindquist block")
+    dt = sorted(times)[len(times) // 2]
     cells = n ** 3
     print(json.dumps({"value": cells / dt, "unit": "cell-updates/s", "cores": threads, "kind": kind,
-                      "sample": sample, "seconds": dt}))
+                      "sample": sample, "seconds_per_step": dt,
+                      "runs_cell_updates_per_s": [cells / t for t in times]}))
 
 
 if __name__ == "__main__":

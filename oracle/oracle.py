@@ -150,11 +150,14 @@ class Oracle:
                              C.c_double(dt), _p(r))
         return dst
 
-    def rk3_step(self, g: Geom, u, dt, bc_mode=0, stage=None, rhs=None, **prm):
+    def rk3_step(self, g: Geom, u, dt, bc_mode=0, stage=None, rhs=None, sources=None, **prm):
         stage = g.zeros() if stage is None else stage
         rhs = g.zeros() if rhs is None else rhs
-        self.lib.orc_rk3_step(*g.cargs(), param_array(**prm), _p(u), _p(stage), _p(rhs), C.c_double(dt),
-                              C.c_int(bc_mode))
+        rho = Si = Sij = S = None
+        if sources is not None:
+            rho, Si, Sij, S = sources
+        self.lib.orc_rk3_step_src(*g.cargs(), param_array(**prm), _p(u), _p(stage), _p(rhs), C.c_double(dt),
+                                  C.c_int(bc_mode), _p(rho), _p(Si), _p(Sij), _p(S))
         return u
 
     def init_flat(self, g: Geom):

@@ -1,6 +1,7 @@
 """granite_main (granite_amd/host/granite_main.cpp) on the CPU: parameter parsing and the
 level geometry it prints (--dry-run touches no GPU) against granite_amd.hierarchy, which is
 pinned to the reference's regrid. The GPU run is in test_gpu_driver.py."""
+import glob
 import os
 import re
 import subprocess
@@ -66,6 +67,76 @@ def test_reports_yaml_errors(tmp_path):
     p.write_text("grid:\n  ncells: [8, 8, 8]\n  ghost_cells: 3\n")
     res = run(str(p), "--dry-run")
     assert res.returncode == 2 and "ghost_cells" in res.stderr
+
+
+REFERENCE_YAML = sorted(glob.glob("/root/reference/benchmarks/**/*.yaml", recursive=True))
+
+
+@pytest.mark.skipif(not REFERENCE_YAML, reason="reference benchmark files not present on this machine")
+@pytest.mark.parametrize("path", REFERENCE_YAML, ids=os.path.basename)
+def test_reference_parameter_files_parse(path):
+    """Every parameter file of the reference's benchmarks reads without a YAML error (the
+    driver may still refuse a setup it does not cover, e.g. two_punctures, with exit 2)."""
+    res = run(path, "--dry-run")
+    assert "YAML parsing error" not in res.stderr, res.stderr
+    assert res.returncode in (0, 2), res.stderr
+
+
+@pytest.fixture(scope="module")
+def yaml_harness(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("yaml") / "yaml_harness"
+    src = os.path.join(HERE, "cpp", "yaml_harness.cpp")
+    inc = os.path.join(os.path.dirname(HERE), "granite_amd", "host")
+    res = subprocess.run(["g++", "-std=c++17", "-O1", "-I", inc, src, "-o", str(exe)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    return str(exe)
+
+
+def test_yaml_block_scalars(yaml_harness, tmp_path):
+    """Folded (>) and literal (|) block scalars with clip, strip (-) and keep (+) chomping,
+    a '#' inside the text, a blank line inside a folded scalar, and the mapping that follows."""
+    p = tmp_path / "b.yaml"
+    p.write_text(
+        "scenario:\n"
+        "  folded: >\n"
+        "    five black holes\n"
+        "    # of equal mass\n"
+        "\n"
+        "    at a pentagon\n"
+        "  literal: |\n"
+        "    line one\n"
+        "      indented\n"
+        "  strip: >-\n"
+        "    a\n"
+        "    b\n"
+        "  keep: |+\n"
+        "    k\n"
+        "\n"
+        "  name: B5  # comment\n"
+        "after: 3\n")
+
+    def get(*keys):
+        res = subprocess.run([yaml_harness, str(p), *keys], capture_output=True, text=True)
+        assert res.returncode == 0, res.stderr
+        return res.stdout.strip()
+
+    assert get("scenario", "folded") == "[five black holes # of equal mass\\nat a pentagon\\n]"
+    assert get("scenario", "literal") == "[line one\\n  indented\\n]"
+    assert get("scenario", "strip") == "[a b]"
+    assert get("scenario", "keep") == "[k\\n\\n]"
+    assert get("scenario", "name") == "[B5]"
+    assert get("after") == "[3]"
+
+
+def test_driver_installed_where_the_reference_launcher_looks():
+    """scripts/run_granite.py:292-297 runs build/bin/granite_main under its default build
+    directory; the build places the driver there too."""
+    if not os.path.exists(B.DRIVER):
+        pytest.skip("driver not built")
+    assert os.path.exists(B.DRIVER_LAUNCHER)
+    res = subprocess.run([B.DRIVER_LAUNCHER, os.path.join(FIX, "gauge_wave_small.yaml"), "--dry-run"],
+                         capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0 and "Grid: 16 x 8 x 8" in res.stdout
 
 
 def test_checkpoint_reader_layout(tmp_path):

@@ -81,11 +81,110 @@ def test_batched_level_against_oracle(engine, oracle):
     lvl.close()
 
 
-def test_level_rejects_mixed_spacing(engine):
+def test_level_with_unequal_spacing(engine):
+    """Blocks of one level whose dx differ (by one ulp, as separately built boxes of one level
+    can; or entirely) run in one launch per spacing, each with its own stencil constants, and
+    come out bit-identical to the same block evolved alone."""
     import granite_amd as ga
 
-    a = ga.GridBlock(0, 0, (8, 8, 8), (0, 0, 0), (1, 1, 1), 4)
-    b = ga.GridBlock(1, 0, (8, 8, 8), (0, 0, 0), (2, 2, 2), 4)
-    with pytest.raises(ga.EngineError) as ei:
-        ga.Level([a, b], engine)
-    assert ei.value.code == -2
+    p = ga.CCZ4Params()
+    h = 0.4
+
+    def box(i, n, lo):
+        # GridBlock's dx = (hi - lo) / n of hi = lo + n * h: h or an ulp either side of it
+        return ga.GridBlock(i, 0, n, lo, [lo[d] + n[d] * h for d in range(3)], 4)
+
+    a = box(0, (8, 10, 12), (-18.31, -18.31, -18.31))   # dx 0.4 - 1ulp, 0.4, 0.4 + 1ulp
+    b = box(1, (8, 14, 8), (-5.79, -5.79, 12.04))       # 0.4, 0.4, 0.4 - 1ulp
+    c = ga.GridBlock(2, 0, (8, 8, 8), (-3.2, -3.2, -3.2), (3.2, 3.2, 3.2), 4)  # dx 0.8
+    dxs = {tuple(x.cellSpacing()) for x in (a, b, c)}
+    say(f"level with spacings {sorted(dxs)}")
+    assert len(dxs) == 3
+    masses, pos = [0.5], [[0.3, 0.2, 0.1]]
+    lvl = ga.Level([a, b, c], engine)
+    lvl.init_brill_lindquist(masses, pos)
+    for _ in range(2):
+        lvl.rk3_step(p, 0.05, pre_fill=ga.BC_OUTFLOW, use_graph=True)
+    lvl.download()
+    for blk in (a, b, c):
+        alone = ga.GridBlock(0, 0, blk.numCells(), blk.lowerCorner(), blk.upperCorner(), 4)
+        one = ga.Level([alone], engine)
+        one.init_brill_lindquist(masses, pos)
+        for _ in range(2):
+            one.rk3_step(p, 0.05, pre_fill=ga.BC_OUTFLOW)
+        one.download()
+        assert np.array_equal(alone.array, blk.array)
+        one.close()
+    lvl.close()
+
+
+def c4_levels(ga, engine):
+    from granite_amd import workloads
+
+    layout = workloads.uneven_amr_levels()
+    out = []
+    for dx, blocks in layout:
+        grids = [ga.GridBlock(i, 0, n, lo, hi, 4) for i, (n, lo, hi) in enumerate(blocks)]
+        out.append((dx, grids))
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_c4_layout_batched_equals_blocks_alone(engine, precision):
+    """The batched-AMR workload itself (granite_amd.workloads.uneven_amr_levels: 250 blocks,
+    edges from {16, 24, 32, 48, 64}, 4 levels): one RK3 step of every level in one stage launch
+    per level; every block (FP64) or every fourth block (mixed) bit-identical to the block
+    evolved alone."""
+    import granite_amd as ga
+
+    from granite_amd import workloads
+
+    p = ga.CCZ4Params()
+    masses, pos = workloads.PUNCTURES
+    levels = c4_levels(ga, engine)
+    assert sum(len(g) for _, g in levels) == 250
+    for li, (dx, grids) in enumerate(levels):
+        lvl = ga.Level(grids, engine)
+        lvl.init_brill_lindquist(masses, pos)
+        say(f"C4 level {li}: {len(grids)} blocks batched, precision {precision}")
+        lvl.rk3_step(p, 0.25 * dx, precision=precision, pre_fill=ga.BC_OUTFLOW, use_graph=True)
+        lvl.download()
+        lvl.close()
+        step = 1 if precision == 0 else 4
+        for i in range(0, len(grids), step):
+            g = grids[i]
+            alone = ga.GridBlock(0, 0, g.numCells(), g.lowerCorner(), g.upperCorner(), 4)
+            one = ga.Level([alone], engine)
+            one.init_brill_lindquist(masses, pos)
+            one.rk3_step(p, 0.25 * dx, precision=precision, pre_fill=ga.BC_OUTFLOW)
+            one.download()
+            assert np.array_equal(alone.array, g.array), (li, i)
+            one.close()
+            g.array = None  # release the host copy
+
+
+def test_c4_layout_against_oracle(engine, oracle):
+    """Four blocks of the batched-AMR workload (the smallest of each level) after one batched
+    RK3 step of their level, against the oracle's step of each block: <= 1e-12 of each tensor
+    group."""
+    import granite_amd as ga
+
+    from granite_amd import workloads
+
+    p = ga.CCZ4Params()
+    masses, pos = workloads.PUNCTURES
+    for li, (dx, grids) in enumerate(c4_levels(ga, engine)):
+        lvl = ga.Level(grids, engine)
+        lvl.init_brill_lindquist(masses, pos)
+        say(f"C4 level {li} vs oracle")
+        lvl.rk3_step(p, 0.25 * dx, pre_fill=ga.BC_OUTFLOW)
+        i = min(range(len(grids)), key=lambda q: int(np.prod(grids[q].numCells())))
+        lvl.download(i)
+        b = grids[i]
+        gg = Geom(b.numCells(), 4, b.lowerCorner(), b.upperCorner())
+        u = oracle.init_bl(gg, masses, pos)
+        oracle.fill_ghosts(gg, u, 2)
+        oracle.rk3_step(gg, u, 0.25 * dx, 0)
+        e = group_rel(b.array, u)
+        assert e.max() < 1e-12, (li, e)
+        lvl.close()

@@ -164,3 +164,74 @@ def test_driver_mixed_precision_matches_binding(engine, tmp_path):
     H.close()
     for L in levels:
         L.close()
+
+
+# the patterns python/granite_analysis/core/parsers.py:33-38 reads these lines with
+RE_NAN_ST = re.compile(r"\[NaN@step=(\d+)\]\s+ST\s+var=(\d+)\s+\((\d+),(\d+),(\d+)\)\s*=\s*(\S+)")
+RE_CFL_W = re.compile(r"\[CFL-(?:WARN|GUARD)\].*CFL=([\d.eE+\-]+)")
+
+
+def _yaml(tmp_path, name, **over):
+    text = open(os.path.join(HERE, "driver", "bl_binary_small.yaml")).read()
+    for key, val in over.items():
+        text = re.sub(rf"(\n\s*{key}:)\s*[^\n#]*", rf"\1 {val} ", text, count=1)
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_driver_nan_line_parses_like_the_reference(engine, tmp_path):
+    """A NaN dissipation strength (applyDissipation only skips sigma <= 0) poisons every
+    interior RHS value in the first step: chi and the lapse are floored, the conformal metric
+    is reset to the identity by the algebraic constraints, and the first non-finite value in
+    the reference's scan order is A~_xx at the first interior cell of the coarsest block. The
+    driver prints it in the reference's format, which the telemetry parser's pattern reads."""
+    if not os.path.exists(B.DRIVER):
+        pytest.skip("driver not built")
+    cfg = _yaml(tmp_path, "nan.yaml", ko_sigma="nan", t_final="0.5", output_interval="1")
+    say("granite_main with ko_sigma = nan")
+    res = subprocess.run([B.DRIVER, cfg, "--no-checkpoints"], capture_output=True, text=True, timeout=120,
+                         cwd=tmp_path)
+    assert res.returncode == 0, res.stderr[-2000:]
+    hits = [m.groups() for m in RE_NAN_ST.finditer(res.stdout)]
+    assert [h[:5] for h in hits] == [("1", "7", "4", "4", "4"), ("2", "7", "4", "4", "4")], res.stdout[-3000:]
+    assert all(h[5].lstrip("-") == "nan" for h in hits)
+    assert "all finite" not in res.stdout
+
+
+def test_driver_cfl_guard_lines(engine, tmp_path):
+    """At CFL 2 the shift driven by the punctures' Gamma-hat exceeds the advective limit in the
+    first coarse steps: the driver prints the reference's [CFL-GUARD] line for every level
+    step above 0.95, in subcycle order, with the values the binding's CFL monitor gives."""
+    import granite_amd as ga
+
+    if not os.path.exists(B.DRIVER):
+        pytest.skip("driver not built")
+    cfg = _yaml(tmp_path, "cfl.yaml", cfl="2.0", t_final="4.0")
+    say("granite_main at CFL 2")
+    res = subprocess.run([B.DRIVER, cfg, "--no-checkpoints"], capture_output=True, text=True, timeout=120,
+                         cwd=tmp_path)
+    assert res.returncode == 0, res.stderr[-2000:]
+    printed = [float(m[1]) for m in RE_CFL_W.finditer(res.stdout)]
+    spheres = puncture_spheres([0.5, 0.5], [[1.5, 0, 0], [-1.5, 0, 0]], 15)
+    geo = tracking_levels([24] * 3, [-12.0 + OFFSET] * 3, [12.0 + OFFSET] * 3, spheres, 15)
+    levels = [ga.Level([ga.GridBlock(0, l, n, lo, hi, 4) for n, lo, hi in lv], engine) for l, lv in enumerate(geo)]
+    H = ga.Hierarchy(levels, engine)
+    H.set_cfl_monitor(True)
+    levels[0].init_brill_lindquist([0.5, 0.5], [[1.5, 0, 0], [-1.5, 0, 0]])
+    for l in range(1, 4):
+        H.prolongate(l)
+    H.fill_ghosts(0)
+    p = ga.CCZ4Params(kappa1=0.02, kappa2=0.0, eta=2.0, ko_sigma=0.1)
+    want = []
+    for _ in range(2):
+        H.step(p, 2.0, ga.PREC_FP64, use_graph=True)
+        vals = H.cfl()
+        assert [lv for lv, _ in vals] == [0, 1, 2, 3, 3, 2, 3, 3, 1, 2, 3, 3, 2, 3, 3]
+        want += [v for _, v in vals if v > 0.95]
+    say(f"CFL lines: driver {printed[:4]}..., binding {want[:4]}...")
+    assert len(want) >= 1 and len(printed) == len(want)
+    assert all(abs(a - b) <= 1e-5 * b for a, b in zip(printed, want))
+    H.close()
+    for L in levels:
+        L.close()

@@ -1,0 +1,195 @@
+"""The pointwise repairs the stage kernels fuse, on states that reach them.
+
+TimeIntegrator::sspRK3Step repairs chi and the lapse after every stage (src/main.cpp:588-608:
+chi non-finite or < 1e-4 -> 1e-4, chi > 1.5 -> 1.5, alpha non-finite or < 1e-6 -> 1e-6),
+resets a conformal metric whose determinant is not finite or <= 1e-10 to the identity after
+the last stage (src/main.cpp:622-645), and computeRHS floors a non-finite or small chi before
+using it (src/spacetime/ccz4.cpp:278-279). A Brill-Lindquist block is seeded with cells and
+cubes that take every one of these branches; the GPU result is compared with the oracle:
+the sets of non-finite values must be equal (a NaN or an infinity poisons the stencils
+around it in any implementation), and every finite value is held to the FP64 tolerance.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Geom, group_rel, normwise_rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+N = 40
+H = 0.25
+
+
+def seeded_state(oracle):
+    """BL puncture (M = 1, at a cell corner) on a 40^3 block at C2 spacing, then:
+    chi cube at 5e-5 (floor), chi cube at 2.0 (ceiling), lapse cube at 1e-7 (floor), single
+    cells with chi = NaN, +Inf, -1 and alpha = NaN, -1, a cube whose conformal metric is
+    1e-4 * identity (det 1e-12: reset), and one cell with gt_xx = NaN (non-finite det)."""
+    half = H * N / 2
+    g = Geom((N, N, N), 4, (-half,) * 3, (half,) * 3)
+    u = oracle.init_bl(g, [1.0], [[0.0, 0.0, 0.0]])
+    s = 4  # first interior index
+
+    def cube(v, c, r, val):
+        k, j, i = (s + x for x in c)
+        u[v, k - r:k + r + 1, j - r:j + r + 1, i - r:i + r + 1] = val
+
+    cube(0, (8, 8, 8), 4, 5e-5)
+    cube(0, (8, 31, 8), 4, 2.0)
+    cube(18, (31, 8, 8), 4, 1e-7)
+    for v in (1, 4, 6):
+        cube(v, (31, 31, 31), 4, 1e-4)
+    for v, c, val in ((0, (20, 8, 30), np.nan), (0, (8, 20, 30), np.inf), (0, (30, 20, 20), -1.0),
+                      (18, (20, 30, 8), np.nan), (18, (14, 30, 30), -1.0), (1, (30, 30, 14), np.nan)):
+        k, j, i = (s + x for x in c)
+        u[v, k, j, i] = val
+    return g, u
+
+
+def compare(got, ref, tol, norm=group_rel):
+    """Equal non-finite sets; finite values within tol (per field)."""
+    bad_g = ~np.isfinite(got)
+    bad_r = ~np.isfinite(ref)
+    assert np.array_equal(bad_g, bad_r), (int(bad_g.sum()), int(bad_r.sum()))
+    a = np.where(bad_r, 0.0, got)
+    b = np.where(bad_r, 0.0, ref)
+    e = norm(a, b)
+    assert e.max() < tol, e
+    return bad_r
+
+
+def test_rhs_on_poisoned_state(engine, oracle):
+    import granite_amd as ga
+
+    g, u = seeded_state(oracle)
+    I = g.interior()
+    say("rhs of the seeded state (NaN, Inf, negative and tiny chi and lapse, degenerate metric)")
+    ev = ga.CCZ4Evolution(ga.CCZ4Params(), engine)
+    blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
+    blk.array[...] = u
+    rhs = ga.GridBlock(1, 0, g.n, g.lo, g.hi, g.ng)
+    ev.computeRHSVacuum(blk, rhs)
+    bad = compare(rhs.array[I], oracle.rhs(g, u)[I], TOL, norm=normwise_rel)
+    assert bad.any()  # the poisoned cells are part of the comparison
+
+
+def test_rk3_step_takes_every_repair_branch(engine, oracle):
+    import granite_amd as ga
+
+    g, u = seeded_state(oracle)
+    blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
+    blk.array[...] = u
+    lvl = ga.Level([blk], engine)
+    lvl.upload()
+    say("one FP64 RK3 step of the seeded state")
+    lvl.rk3_step(ga.CCZ4Params(), 0.0625, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
+    lvl.download()
+    ref = u.copy()
+    oracle.fill_ghosts(g, ref, 2)
+    oracle.rk3_step(g, ref, 0.0625, 0)
+    compare(blk.array, ref, TOL)
+    s = 4
+    at = lambda v, c: ref[v, s + c[0], s + c[1], s + c[2]]  # noqa: E731
+    # each branch was taken: the oracle's result holds the repaired values there
+    assert at(0, (8, 8, 8)) == 1e-4 and at(0, (8, 31, 8)) == 1.5 and at(18, (31, 8, 8)) == 1e-6
+    assert at(0, (20, 8, 30)) == 1e-4 and at(0, (8, 20, 30)) == 1e-4 and at(0, (30, 20, 20)) == 1e-4
+    assert at(18, (20, 30, 8)) == 1e-6 and at(18, (14, 30, 30)) == 1e-6
+    assert [at(v, (31, 31, 31)) for v in range(1, 7)] == [1, 0, 0, 1, 0, 1]
+    assert [at(v, (30, 30, 14)) for v in range(1, 7)] == [1, 0, 0, 1, 0, 1]
+    lvl.close()
+
+
+def test_mixed_form_routes_nonfinite_tiles_to_fp64(engine, oracle):
+    """The mixed form classifies tiles whose smallest chi is below the threshold or not
+    finite (the NaN chi cell, the 5e-5 cube) as FP64 tiles and runs the rest in FP32: the same
+    non-finite set as the FP64 form, the same repaired constants, and finite values within the
+    mixed form's tolerance of the FP64 form's."""
+    import granite_amd as ga
+
+    g, u = seeded_state(oracle)
+    out = {}
+    for prec in (ga.PREC_FP64, ga.PREC_MIXED):
+        blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
+        blk.array[...] = u
+        lvl = ga.Level([blk], engine)
+        lvl.upload()
+        say(f"seeded state, one RK3 step, precision {prec}")
+        lvl.rk3_step(ga.CCZ4Params(), 0.0625, precision=prec, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
+        lvl.download()
+        out[prec] = blk.array.copy()
+        lvl.close()
+    a, b = out[ga.PREC_MIXED], out[ga.PREC_FP64]
+    assert np.array_equal(~np.isfinite(a), ~np.isfinite(b))
+    # the repaired values are the same constants in both forms
+    s = 4
+    for v, c, val in ((0, (8, 8, 8), 1e-4), (0, (8, 31, 8), 1.5), (18, (31, 8, 8), 1e-6), (0, (20, 8, 30), 1e-4),
+                      (0, (8, 20, 30), 1e-4), (18, (20, 30, 8), 1e-6)):
+        assert a[v, s + c[0], s + c[1], s + c[2]] == val
+    fin = np.isfinite(b)
+    e = group_rel(np.where(fin, a, 0.0), np.where(fin, b, 0.0))
+    assert e.max() < 1e-4, e
+
+
+def test_first_nonfinite_and_cfl(engine, oracle):
+    """The NaN scan's first hit in the reference's order (field, then k, j, i over the
+    interior; cell indices counted with ghosts) and the advective CFL guard's maximum."""
+    import granite_amd as ga
+
+    g = Geom((18, 11, 9), 4, (-2.0, -1.0, -1.0), (2.5, 1.75, 1.25))
+    u = oracle.init_bl(g, [1.0], [[0.1, 0.0, 0.0]])
+    rng = np.random.default_rng(3)
+    for v in (19, 20, 21):
+        u[v] = rng.standard_normal(u[v].shape) * 0.3
+    blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
+    blk.array[...] = u
+    lvl = ga.Level([blk], engine)
+    lvl.upload()
+    say("first non-finite on a finite state, CFL maximum")
+    assert lvl.first_nonfinite() is None
+    dt = 0.07
+    I = g.interior()
+    local = sum(np.abs(u[19 + d][I[1:]]) * dt / g.dx[d] for d in range(3))
+    assert abs(lvl.cfl(dt) - local.max()) <= 1e-15 * local.max()
+    # poison: ghosts are not scanned, later fields and later cells lose to earlier ones
+    u[3, 0, 0, 0] = np.nan           # ghost
+    u[7, 6, 5, 9] = np.inf           # field 7
+    u[5, 10, 4, 12] = -np.inf        # field 5, k = 10
+    u[5, 6, 9, 4] = np.nan           # field 5, k = 6: the first in the scan order
+    u[19, 5, 5, 5] = np.nan          # NaN shift: ignored by the CFL maximum
+    blk.array[...] = u
+    lvl.upload()
+    say("first non-finite on a poisoned state")
+    b, v, i, j, k, val = lvl.first_nonfinite()
+    assert (b, v, i, j, k) == (0, 5, 4, 9, 6) and np.isnan(val)
+    local = sum(np.abs(u[19 + d][I[1:]]) * dt / g.dx[d] for d in range(3))
+    fin = np.where(np.isnan(local), 0.0, local)
+    assert np.isnan(local).sum() == 1
+    assert abs(lvl.cfl(dt) - fin.max()) <= 1e-15 * fin.max()
+    lvl.close()
+
+
+def test_gauge_wave_initial_data(engine, oracle):
+    """setGaugeWaveData on the device (sin, pow, sqrt of the device's math library) against
+    the oracle's (the host's libm): equal to a few ulp."""
+    import granite_amd as ga
+
+    g = Geom((24, 6, 5), 4, (0.0, 0.0, 0.0), (1.0, 0.25, 0.2))
+    blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
+    lvl = ga.Level([blk], engine)
+    say("gauge-wave initial data")
+    lvl.init_gauge_wave(0.1, 1.0)
+    lvl.download()
+    ref = oracle.init_gauge_wave(g, 0.1, 1.0)
+    assert np.array_equal(blk.array == 0.0, ref == 0.0)
+    assert np.max(np.abs(blk.array - ref)) <= 4e-16 * np.max(np.abs(ref))
+    lvl.close()

@@ -75,3 +75,36 @@ def test_mirror_against_oracle(driver, oracle, tmp_path):
     oracle.rk3_step(g, ref, 0.0625, 0)
     oracle.rk3_step(g, ref, 0.0625, 0)
     assert group_rel(s, ref).max() < 1e-12
+
+
+REFERENCE = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFERENCE, "include", "granite", "core", "grid.hpp")),
+                    reason="reference sources not present on this machine")
+def test_mirror_with_the_reference_types(tmp_path):
+    """The mirror instantiated with GRANITE's own granite::GridBlock and
+    granite::spacetime::CCZ4Params (the reference's headers and src/core/grid.cpp, compiled
+    where they lie): every method compiles against the real types, the geometry handed to the
+    engine is the block's own, and without a device the engine's constructor throws the
+    runtime_error the reference's callers would see."""
+    import granite_amd
+
+    exe = str(tmp_path / "reference_types_mirror")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(REFERENCE, "include"), "-I",
+           os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "granite_amd", "host"),
+           os.path.join(ROOT, "tests", "cpp", "reference_types_mirror.cpp"),
+           os.path.join(REFERENCE, "src", "core", "grid.cpp"), "-L", os.path.join(ROOT, "granite_amd"),
+           "-lgranite_ccz4", "-Wl,-rpath," + os.path.join(ROOT, "granite_amd"), "-o", exe]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    try:
+        granite_amd.Engine(0)
+        have_gpu = True
+    except granite_amd.EngineError:
+        have_gpu = False
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    if have_gpu:
+        assert out.returncode == 0 and "ran on the device" in out.stdout, out.stdout + out.stderr
+    else:
+        assert out.returncode == 1 and "no usable gfx950 device" in out.stdout, out.stdout + out.stderr

@@ -62,3 +62,17 @@ def test_amr_transfer_matches_reference(oracle):
     rest = oracle.restrict(gf, uf, gc, uc.copy())
     assert np.array_equal(prol[:4], z["prolongated"])
     assert np.array_equal(rest[:4], z["restricted"])
+
+
+@pytest.mark.parametrize("sigma", [float("nan"), -0.1, 0.0])
+def test_dissipation_switch_matches_reference(oracle, reference, sigma):
+    """applyDissipation returns early only when sigma <= 0 (src/spacetime/ccz4.cpp:873-874): a
+    NaN sigma dissipates (and poisons every interior RHS value), a negative or zero one does
+    not; the oracle's RHS agrees with the compiled reference's in each case."""
+    rec, _ = load("bl_puncture")
+    g, u = make_input(oracle, rec)
+    I = g.interior()
+    mine = oracle.rhs(g, u, ko_sigma=sigma)[I]
+    ref = reference.rhs(g, u, ko_sigma=sigma)[I]
+    assert np.array_equal(mine, ref, equal_nan=True)
+    assert np.isnan(mine).all() == (sigma != sigma)
