@@ -63,6 +63,7 @@ struct gr_level {
     long maxstride = 0;
     double* partial = nullptr; // reduction partials
     double* red = nullptr;     // reduction result (1 double)
+    double* cfl_part = nullptr; // CFL-guard partials and result (on first use)
     double* bh = nullptr;      // puncture masses/positions for initial data
     double chi_fp64_below = 0.2;
     unsigned generation = 0;   // bumped whenever captured launches of the level go stale
@@ -460,6 +461,7 @@ int gr_level_destroy(gr_level* L) {
     if (L->mom) (void)hipFree(L->mom);
     if (L->partial) (void)hipFree(L->partial);
     if (L->red) (void)hipFree(L->red);
+    if (L->cfl_part) (void)hipFree(L->cfl_part);
     if (L->bh) (void)hipFree(L->bh);
     delete L;
     return GR_OK;
@@ -894,14 +896,14 @@ int gr_level_cfl(gr_level* L, double dt, double* max_cfl) {
     int rc = use_device(e);
     if (rc) return rc;
     if (!max_cfl || !std::isfinite(dt)) return fail(e, GR_ERR_ARG, "null output or non-finite dt");
-    double* part = nullptr;
-    GR_HIP(e, hipMalloc((void**)&part, (L->blk.size() * kCflPartsPerBlock + 1) * sizeof(double)));
+    if (!L->cfl_part)
+        GR_HIP(e, hipMalloc((void**)&L->cfl_part, (L->blk.size() * kCflPartsPerBlock + 1) * sizeof(double)));
+    double* part = L->cfl_part;
     hipError_t st = enqueue_cfl(L, dt, part, kCflPartsPerBlock, part + L->blk.size() * kCflPartsPerBlock);
     if (st == hipSuccess)
         st = hipMemcpyAsync(max_cfl, part + L->blk.size() * kCflPartsPerBlock, sizeof(double), hipMemcpyDeviceToHost,
                             e->stream);
     if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
-    (void)hipFree(part);
     if (st != hipSuccess) return hipfail(e, st, "advective CFL reduction");
     return GR_OK;
 }

@@ -14,10 +14,13 @@
 // Ricci tensor, A~A~ and the lapse Laplacian factored once per cell (same mathematics,
 // different rounding order; parity is normwise, not bitwise).
 //
-// Fields are streamed in the order beta, chi, alpha, conformal metric, Gamma-hat, K, Theta,
+// Fields are streamed in the order beta, conformal metric, chi, alpha, K, Gamma-hat, Theta,
 // A~ so that each field's stencil results are folded into a small set of accumulators as
-// soon as they are produced; only the first derivatives of the metric, chi, alpha and beta,
-// and the second derivatives of chi and alpha, stay live across fields.
+// soon as they are produced: the Christoffel symbols exist before chi and alpha are staged,
+// so their second derivatives are consumed at once, and every term that needs the
+// Christoffel symbols is formed before K is staged; across the later fields only the
+// shift's first derivatives, the lapse's Hessian, the Ricci tensor and the Gamma-hat
+// accumulators stay live.
 #pragma once
 
 #ifndef __HIPCC__
@@ -43,7 +46,7 @@ constexpr double PI = 3.14159265358979323846;
 
 // Order in which ccz4_cell stages the fields (Stager::field calls). A stager may rely on it
 // to prefetch the next fields while the current one is being evaluated.
-constexpr int FIELD_ORDER[NV] = {19, 20, 21, 0, 18, 1, 2, 3, 4, 5, 6, 13, 14, 15, 16, 17, 7, 8, 9, 10, 11, 12};
+constexpr int FIELD_ORDER[NV] = {19, 20, 21, 1, 2, 3, 4, 5, 6, 0, 18, 13, 14, 15, 16, 17, 7, 8, 9, 10, 11, 12};
 __host__ __device__ constexpr int field_pos(int v) {
     int p = 0;
     while (FIELD_ORDER[p] != v) ++p;
@@ -256,16 +259,6 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
     for (int i = 0; i < 3; ++i)
         GA[i] += (R(1) / R(3)) * (gu[sym(i, 0)] * ddiv[0] + gu[sym(i, 1)] * ddiv[1] + gu[sym(i, 2)] * ddiv[2]);
 
-    // ---- chi (ccz4.cpp:580)
-    R dchi[3], ddchi[6];
-    eval_field<true, KO_EXACT>(S.field(CHI), K, S.strides(), be, w, dchi, ddchi, adv, ko);
-    O.put(CHI, (R(2) / R(3)) * chi * (alpha * Ktr - divb) + adv, ko, chi_raw);
-
-    // ---- lapse (ccz4.cpp:820)
-    R dal[3], ddal[6];
-    eval_field<true, KO_EXACT>(S.field(ALPHA), K, S.strides(), be, w, dal, ddal, adv, ko);
-    O.put(ALPHA, R(-2) * alpha * Ktr + adv, ko, alpha);
-
     // ---- conformal metric: keeps d gt; second derivatives give Ricci term 1
     //      -1/2 g^lm d_l d_m gt_ij (ccz4.cpp:470-476); RHS (ccz4.cpp:586-607)
     R dgt[6][3], Rt[6];
@@ -284,6 +277,24 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
         O.put(GXX + p, R(-2) * alpha * At[p] + L - (R(2) / R(3)) * gt[p] * divb, ko, gt[p]);
     }
 
+    // ---- the remaining uses of the shift's first derivatives, formed now so that they die:
+    //      the Lie terms of A~ (ccz4.cpp:708-715) and of Gamma-hat's T7 (ccz4.cpp:795-808)
+    R LieA[6];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        const int a = sym_a(p), b = sym_b(p);
+        R L = R(0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) L += At[sym(a, q)] * dbe[q][b] + At[sym(b, q)] * dbe[q][a];
+        LieA[p] = L - (R(2) / R(3)) * At[p] * divb;
+        pin(LieA[p]);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        GA[q] += -(Gh[0] * dbe[q][0] + Gh[1] * dbe[q][1] + Gh[2] * dbe[q][2]) + (R(2) / R(3)) * Gh[q] * divb;
+        pin(GA[q]);
+    }
+
     // ---- Christoffel symbols chr[i][jk] (ccz4.cpp:413-444)
     R chr[3][6];
 #pragma unroll
@@ -298,16 +309,13 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
             pin(chr[i][p]);
         }
 
-    // ---- covariant Hessian of the lapse D_i D_j alpha (ccz4.cpp:678-680) and the
-    //      conformal-factor Ricci R^chi (ccz4.cpp:518-541), folded in as soon as the
-    //      Christoffels exist so that the raw second derivatives of chi and alpha die here
-    R DDa[6];
-#pragma unroll
-    for (int p = 0; p < 6; ++p) {
-        DDa[p] = ddal[p] - chr[0][p] * dal[0] - chr[1][p] * dal[1] - chr[2][p] * dal[2];
-        pin(DDa[p]);
-    }
+    // ---- chi (ccz4.cpp:580); its second derivatives give the conformal-factor Ricci R^chi
+    //      (ccz4.cpp:518-541) at once, with the Christoffels already formed
+    R dchi[3];
     {
+        R ddchi[6];
+        eval_field<true, KO_EXACT>(S.field(CHI), K, S.strides(), be, w, dchi, ddchi, adv, ko);
+        O.put(CHI, (R(2) / R(3)) * chi * (alpha * Ktr - divb) + adv, ko, chi_raw);
         R lapchi = R(0), gradchi2 = R(0);
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -324,6 +332,20 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
             const R Dab = ddchi[p] - chr[0][p] * dchi[0] - chr[1][p] * dchi[1] - chr[2][p] * dchi[2];
             Rt[p] += c1 * (Dab + gt[p] * lapchi) - c2 * (dchi[a] * dchi[b] - R(3) * gt[p] * gradchi2);
             pin(Rt[p]);
+        }
+    }
+
+    // ---- lapse (ccz4.cpp:820); its second derivatives give the covariant Hessian
+    //      D_i D_j alpha (ccz4.cpp:678-680) at once
+    R dal[3], DDa[6];
+    {
+        R ddal[6];
+        eval_field<true, KO_EXACT>(S.field(ALPHA), K, S.strides(), be, w, dal, ddal, adv, ko);
+        O.put(ALPHA, R(-2) * alpha * Ktr + adv, ko, alpha);
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            DDa[p] = ddal[p] - chr[0][p] * dal[0] - chr[1][p] * dal[1] - chr[2][p] * dal[2];
+            pin(DDa[p]);
         }
     }
 
@@ -354,10 +376,10 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
         }
     }
 
-    // ---- K (ccz4.cpp:621-652); its gradient completes T2 of Gamma-hat
+    // ---- lapse Laplacian D^2 alpha (ccz4.cpp:621-648), formed while the first derivatives of
+    //      chi and alpha are live
+    R plap;
     {
-        R dK[3];
-        eval_field<false, KO_EXACT>(S.field(KTR), K, S.strides(), be, w, dK, dd, adv, ko);
         R lap = R(0), cga = R(0);
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -366,7 +388,45 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
             lap += f * gu[q] * DDa[q];
             cga += gu[q] * (l == m ? dchi[l] * dal[l] : dchi[l] * dal[m] + dchi[m] * dal[l]);
         }
-        const R plap = chi * lap - R(0.5) * cga;
+        plap = chi * lap - R(0.5) * cga;
+        pin(plap);
+    }
+
+    // ---- remaining Ricci terms (ccz4.cpp:486-515), formed as soon as the Christoffels exist
+    //      (so that they die before the next fields are staged), factored per k with
+    //      G_k = gt_kn chr^n:
+    //   sum_k (Gh^k - g^lm chr^k_lm) G_k,ij + sum_l chr^k_li H_klj + chr^k_lj H_kli,
+    //   H_klj = g^lm G_k,jm
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        R Gk[6];
+#pragma unroll
+        for (int p = 0; p < 6; ++p) Gk[p] = gt[sym(k, 0)] * chr[0][p] + gt[sym(k, 1)] * chr[1][p] + gt[sym(k, 2)] * chr[2][p];
+        R Gd = R(0);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Gd += ((q == 0 || q == 3 || q == 5) ? R(1) : R(2)) * gu[q] * chr[k][q];
+        const R cg = Gh[k] - Gd;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) Rt[p] += cg * Gk[p];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            R H[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) H[j] = gu[sym(l, 0)] * Gk[sym(j, 0)] + gu[sym(l, 1)] * Gk[sym(j, 1)] + gu[sym(l, 2)] * Gk[sym(j, 2)];
+#pragma unroll
+            for (int p = 0; p < 6; ++p) {
+                const int a = sym_a(p), b = sym_b(p);
+                Rt[p] += chr[k][sym(l, a)] * H[b] + chr[k][sym(l, b)] * H[a];
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) pin(Rt[p]);
+
+    // ---- K (ccz4.cpp:621-652); its gradient completes T2 of Gamma-hat
+    {
+        R dK[3];
+        eval_field<false, KO_EXACT>(S.field(KTR), K, S.strides(), be, w, dK, dd, adv, ko);
         const R matter = HAS_SRC ? R(4) * R(PI) * alpha * (R(src.rho) + R(src.S)) : R(0);
         O.put(KTR,
               -plap + alpha * (AA + Ktr * Ktr / R(3)) + matter + R(P.kappa1) * (R(1) - R(P.kappa2)) * alpha * Theta + adv,
@@ -394,39 +454,18 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
             Rt[p] += R(0.5) * (gt[sym(q, a)] * d1[b] + gt[sym(q, b)] * d1[a]);
             pin(Rt[p]);
         }
-        const R T7 = -(Gh[0] * dbe[q][0] + Gh[1] * dbe[q][1] + Gh[2] * dbe[q][2]) + advq + (R(2) / R(3)) * Gh[q] * divb;
-        O.put(GHX + q, GA[q] + T7, koq, Gh[q]);
+        O.put(GHX + q, GA[q] + advq, koq, Gh[q]);
     }
 
-    // ---- remaining Ricci terms (ccz4.cpp:486-515), factored per k with G_k = gt_kn chr^n:
-    //   sum_k (Gh^k - g^lm chr^k_lm) G_k,ij + sum_l chr^k_li H_klj + chr^k_lj H_kli,
-    //   H_klj = g^lm G_k,jm
+    // ---- the Ricci tensor is complete: its trace for Theta, and -D_i D_j alpha + alpha R_ij
+    //      for A~ (ccz4.cpp:654-660,674-680), so that Rt and the lapse Hessian die here
+    R Rs = R(0), XA[6];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        R Gk[6];
-#pragma unroll
-        for (int p = 0; p < 6; ++p) Gk[p] = gt[sym(k, 0)] * chr[0][p] + gt[sym(k, 1)] * chr[1][p] + gt[sym(k, 2)] * chr[2][p];
-        R Gd = R(0);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Gd += ((q == 0 || q == 3 || q == 5) ? R(1) : R(2)) * gu[q] * chr[k][q];
-        const R cg = Gh[k] - Gd;
-#pragma unroll
-        for (int p = 0; p < 6; ++p) Rt[p] += cg * Gk[p];
-#pragma unroll
-        for (int l = 0; l < 3; ++l) {
-            R H[3];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) H[j] = gu[sym(l, 0)] * Gk[sym(j, 0)] + gu[sym(l, 1)] * Gk[sym(j, 1)] + gu[sym(l, 2)] * Gk[sym(j, 2)];
-#pragma unroll
-            for (int p = 0; p < 6; ++p) {
-                const int a = sym_a(p), b = sym_b(p);
-                Rt[p] += chr[k][sym(l, a)] * H[b] + chr[k][sym(l, b)] * H[a];
-            }
-        }
+    for (int q = 0; q < 6; ++q) {
+        Rs += ((q == 0 || q == 3 || q == 5) ? R(1) : R(2)) * gu[q] * Rt[q];
+        XA[q] = -DDa[q] + alpha * Rt[q];
+        pin(XA[q]);
     }
-    R Rs = R(0);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) Rs += ((q == 0 || q == 3 || q == 5) ? R(1) : R(2)) * gu[q] * Rt[q];
     Rs *= chi;
     pin(Rs);
 
@@ -446,19 +485,14 @@ __host__ __device__ GR_INLINE void ccz4_cell(Stager& S, Sink& O, const Stencil<R
     for (int p = 0; p < 6; ++p) {
         const int a = sym_a(p), b = sym_b(p);
         eval_field<false, KO_EXACT>(S.field(AXX + p), K, S.strides(), be, w, d1, dd, adv, koA[p]);
-        const R Sij = HAS_SRC ? R(src.Sij[p]) : R(0);
-        const R t1 = chi * (-DDa[p] + alpha * (Rt[p] - R(8) * R(PI) * Sij));
+        const R t1 = HAS_SRC ? chi * (XA[p] - alpha * R(8) * R(PI) * R(src.Sij[p])) : chi * XA[p];
         R Aub[3]; // A^q_b = g^qr A_rb
 #pragma unroll
         for (int q = 0; q < 3; ++q)
             Aub[q] = gu[sym(q, 0)] * At[sym(0, b)] + gu[sym(q, 1)] * At[sym(1, b)] + gu[sym(q, 2)] * At[sym(2, b)];
         const R Asq = At[sym(a, 0)] * Aub[0] + At[sym(a, 1)] * Aub[1] + At[sym(a, 2)] * Aub[2];
         const R t2 = alpha * (Ktr * At[p] - R(2) * Asq);
-        R L = adv;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) L += At[sym(a, q)] * dbe[q][b] + At[sym(b, q)] * dbe[q][a];
-        L -= (R(2) / R(3)) * At[p] * divb;
-        raw[p] = t1 + t2 + L;
+        raw[p] = t1 + t2 + (adv + LieA[p]);
         pin(raw[p]);
     }
     {

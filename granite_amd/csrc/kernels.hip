@@ -46,6 +46,13 @@
 #ifndef GR_LDS_PAD
 #define GR_LDS_PAD 1
 #endif
+// LDS layout of the FP64 parts: 0 = rows padded to a conflict-free pitch, lanes x-fastest
+// (55 KB for two buffers of an 8x4x4 part: two workgroups per CU); 1 = unpadded rows,
+// planes padded, lanes interleaved over y and z (38 KB: room for four workgroups per CU,
+// i.e. two waves per SIMD when the registers allow it)
+#ifndef GR_FP64_LAYOUT
+#define GR_FP64_LAYOUT 0
+#endif
 #ifndef GR_MIXED_PREFETCH
 #define GR_MIXED_PREFETCH 1
 #endif
@@ -76,6 +83,15 @@ __device__ __forceinline__ double gload(const gdouble* base, unsigned off) {
 }
 __device__ __forceinline__ void gstore(gdouble* base, unsigned off, double v) {
     *(gdouble*)((gchar*)base + off) = v;
+}
+// the same load with the offset made opaque at the point of use, so that the 32-bit offset
+// reaches the load itself (scalar base + 32-bit vector offset, no 64-bit address add per
+// load) instead of a 64-bit copy of it hoisted out of the loop over fields
+__device__ __forceinline__ double gload_sv(const gdouble* base, unsigned off) {
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("" : "+v"(off));
+#endif
+    return *(const gdouble*)((const gchar*)base + off);
 }
 
 namespace {
@@ -159,13 +175,68 @@ constexpr int conflict_free_pitch(int bx0, int by, int esize) {
     return bx0;
 }
 
-template <int TX, int TY, int TZ, bool PAD = true, int ESIZE = 8>
+// Lane -> cell maps of a tile's threads. Layout 0: x fastest, then y, then z. Layout 1
+// (TX = 8, TY even, TZ a multiple of 4): lane bits 0-2 are x, bit 3 the low bit of y, bits
+// 4-5 the low bits of z, and the wave index supplies the rest of y, then of z. A 64-bit LDS
+// read is served as two passes of 32 lanes; under layout 1 each pass holds 8 x-neighbours of
+// 2 rows and 2 planes, so unpadded rows (pitch TX + 8 = 16 doubles = 32 banks) stay
+// conflict-free once a plane spans 16 banks modulo 64 (a pad of 8 doubles per plane).
+template <int TX, int TY, int TZ, int LAYOUT>
+struct LaneMap {
+    static_assert(LAYOUT == 0 || (TX == 8 && TY % 2 == 0 && TZ % 4 == 0), "layout 1 needs 8 x even x 4k parts");
+    __host__ __device__ static constexpr void cell(int tid, int& x, int& y, int& z) {
+        if (LAYOUT == 0) {
+            x = tid % TX;
+            y = (tid / TX) % TY;
+            z = tid / (TX * TY);
+        } else {
+            const int w = tid >> 6;
+            x = tid & 7;
+            y = ((tid >> 3) & 1) + 2 * (w % (TY / 2));
+            z = ((tid >> 4) & 3) + 4 * (w / (TY / 2));
+        }
+    }
+};
+
+// whether every 32-lane (64-bit elements) or 64-lane (32-bit) pass of every wave reads its
+// cells from distinct banks at row pitch P and plane stride Q (elements of esize bytes)
+template <int TX, int TY, int TZ, int LAYOUT>
+constexpr bool lds_conflict_free(int P, int Q, int esize) {
+    const int words = esize / 4, lanes = 64 / words, NT = TX * TY * TZ;
+    for (int base = 0; base < NT; base += lanes) {
+        bool used[64] = {};
+        for (int t = base; t < base + lanes && t < NT; ++t) {
+            int x = 0, y = 0, z = 0;
+            LaneMap<TX, TY, TZ, LAYOUT>::cell(t, x, y, z);
+            const int a = z * Q + y * P + x;
+            for (int w = 0; w < words; ++w) {
+                const int bank = (a * words + w) % 64;
+                if (used[bank]) return false;
+                used[bank] = true;
+            }
+        }
+    }
+    return true;
+}
+
+// smallest plane stride >= P * BY that is conflict-free at row pitch P
+template <int TX, int TY, int TZ, int LAYOUT>
+constexpr int conflict_free_plane(int P, int BY, int esize) {
+    for (int q = P * BY; q < P * BY + 64; ++q)
+        if (lds_conflict_free<TX, TY, TZ, LAYOUT>(P, q, esize)) return q;
+    return P * BY;
+}
+
+template <int TX, int TY, int TZ, bool PAD = true, int ESIZE = 8, int LAYOUT = 0>
 struct TileGeom {
     static constexpr int NT = TX * TY * TZ;
     static constexpr int BX0 = TX + 2 * HALO, BY = TY + 2 * HALO, BZ = TZ + 2 * HALO;
-    // LDS row pitch (elements): conflict-free centre reads
-    static constexpr int BX = (GR_LDS_PAD && PAD) ? conflict_free_pitch<TX, TY>(BX0, BY, ESIZE) : BX0;
-    static constexpr int NLDS = BX * BY * BZ;
+    using Lanes = LaneMap<TX, TY, TZ, LAYOUT>;
+    // LDS row pitch and plane stride (elements): conflict-free centre reads
+    static constexpr int BX = (LAYOUT == 0 && GR_LDS_PAD && PAD) ? conflict_free_pitch<TX, TY>(BX0, BY, ESIZE) : BX0;
+    static constexpr int PLANE = LAYOUT == 0 ? BX * BY : conflict_free_plane<TX, TY, TZ, LAYOUT>(BX, BY, ESIZE);
+    static_assert(LAYOUT == 0 || lds_conflict_free<TX, TY, TZ, LAYOUT>(BX, PLANE, ESIZE), "no conflict-free plane");
+    static constexpr int NLDS = PLANE * BZ;
     // sub-boxes of the halo region that some stencil reads (x0,x1,y0,y1,z0,z1)
     static constexpr int NBOX = 7;
     static constexpr int box[NBOX][6] = {
@@ -183,7 +254,7 @@ struct TileGeom {
     static constexpr int start(int b) { return b == 0 ? 0 : start(b - 1) + count(b - 1); }
     static constexpr int TOTAL = start(NBOX - 1) + count(NBOX - 1);
     __host__ __device__ static constexpr int lds(int x, int y, int z) {
-        return ((z + HALO) * BY + (y + HALO)) * BX + (x + HALO);
+        return (z + HALO) * PLANE + (y + HALO) * BX + (x + HALO);
     }
     // tile-relative coordinates of halo entry p (boxes in order, x fastest inside a box);
     // unchanged for p >= TOTAL. The boxes are visited at compile time, so their extents
@@ -215,7 +286,7 @@ struct TileGeom {
 // with that field's prefetch.
 template <int TX, int TY, int TZ, int MODE, int PF>
 struct TileIO {
-    using T = TileGeom<TX, TY, TZ>;
+    using T = TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>;
     static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
     static constexpr int LBUF = T::NLDS + 1; // one LDS buffer incl. a discard slot
 
@@ -260,7 +331,7 @@ struct TileIO {
         const int slot = pos % PF;
         const gdouble* f = in + v * stride;
 #pragma unroll
-        for (int i = 0; i < NL; ++i) buf[slot][i] = gload(f, goff[i]);
+        for (int i = 0; i < NL; ++i) buf[slot][i] = gload_sv(f, goff[i]);
         if (MODE != MODE_RHS) unb[slot] = gload(un + v * stride, cb);
     }
 
@@ -280,7 +351,7 @@ struct TileIO {
 #pragma unroll
         for (int a = 0; a < 6; ++a) g[a] = gload(in + (GXX + a) * stride, cb);
     }
-    __device__ FixedStrides<1, T::BX, T::BX * T::BY> strides() const { return {}; }
+    __device__ FixedStrides<1, T::BX, T::PLANE> strides() const { return {}; }
     __device__ const double* field(int v) {
         const int pos = field_pos(v);
         __syncthreads(); // field pos is in its buffer; every thread is done with field pos-1
@@ -376,7 +447,7 @@ struct TileIO32 {
         const int slot = pos % PF;
         const gdouble* f = in + v * stride;
 #pragma unroll
-        for (int i = 0; i < NL; ++i) buf[slot][i] = gload(f, goff[i]);
+        for (int i = 0; i < NL; ++i) buf[slot][i] = gload_sv(f, goff[i]);
         refb[slot] = f[ridx];
         inb[slot] = gload(f, cb);
         if (MODE != MODE_RHS) unb[slot] = gload(un + v * stride, cb);
@@ -400,7 +471,7 @@ struct TileIO32 {
 #pragma unroll
         for (int a = 0; a < 6; ++a) g[a] = gload(in + (GXX + a) * stride, cb);
     }
-    __device__ FixedStrides<1, T::BX, T::BX * T::BY> strides() const { return {}; }
+    __device__ FixedStrides<1, T::BX, T::PLANE> strides() const { return {}; }
     __device__ const float* field(int v) {
         const int pos = field_pos(v);
         __syncthreads();
@@ -550,10 +621,11 @@ __device__ __forceinline__ void load_sources(const Sources& S, long cidx, CellSo
 // z = (part / GR_FP64_SPLIT) * TZ inside the tile
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R, int part, double* lds) {
-    using T = TileGeom<TX, TY, TZ>;
+    using T = TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>;
     const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
-    const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
+    int lx = 0, ly = 0, lz = 0;
+    T::Lanes::cell(tid, lx, ly, lz);
     const int x0 = R.tix * GR_TILE_X + (part % GR_FP64_SPLIT) * TX;
     const int z0 = R.tiz * GR_TILE_Z + (part / GR_FP64_SPLIT) * TZ;
     const int ix = x0 + lx, iy = R.tiy * TY + ly, iz = z0 + lz; // interior-relative
@@ -593,7 +665,7 @@ __device__ __forceinline__ bool part_outside(const TileRef& R, int part) {
 // one part of one tile per workgroup (FP64_PARTS consecutive workgroups per tile)
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
-    using T = TileGeom<TX, TY, TZ>;
+    using T = TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>;
     __shared__ double lds[2 * (T::NLDS + 1)];
     const int slot = tile_of_block(blockIdx.x, A.ntiles * FP64_PARTS, A.xcd_remap);
     const int tile = slot / FP64_PARTS, part = slot % FP64_PARTS;
@@ -608,7 +680,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(St
 // (every thread takes the same trip counts, so the barriers are uniform)
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_scan(StageArgs A) {
-    using T = TileGeom<TX, TY, TZ>;
+    using T = TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>;
     __shared__ double lds[2 * (T::NLDS + 1)];
     __shared__ int list[T::NT];
     __shared__ int nlist;
@@ -637,8 +709,12 @@ static_assert(GR_TILE_X % GR_FP64_SPLIT == 0 && GR_TILE_Z % GR_FP64_ZSPLIT == 0,
 constexpr int TILE_X = GR_TILE_X, TILE_Y = GR_TILE_Y, TILE_Z = GR_TILE_Z;
 constexpr int FP64_TX = GR_TILE_X / GR_FP64_SPLIT, FP64_TY = GR_TILE_Y, FP64_TZ = GR_TILE_Z / GR_FP64_ZSPLIT;
 static_assert(!GR_LDS_PAD || FP64_TX != 8 || TILE_X != 16 || TILE_Y != 4 ||
-                  (TileGeom<FP64_TX, FP64_TY, FP64_TZ>::BX == 24 && TileGeom<TILE_X, TILE_Y, TILE_Z, true, 4>::BX == 48),
-              "LDS pitches of the default tiles: 24 (FP64 rows of 8) and 48 (FP32 rows of 16)");
+                  ((GR_FP64_LAYOUT ? (TileGeom<FP64_TX, FP64_TY, FP64_TZ, true, 8, 1>::BX == 16 &&
+                                      TileGeom<FP64_TX, FP64_TY, FP64_TZ, true, 8, 1>::PLANE == 200)
+                                   : TileGeom<FP64_TX, FP64_TY, FP64_TZ>::BX == 24) &&
+                   TileGeom<TILE_X, TILE_Y, TILE_Z, true, 4>::BX == 48),
+              "LDS layouts of the default tiles: FP64 rows of 16 and planes of 200 (layout 1) or rows of 24 "
+              "(layout 0); FP32 rows of 48");
 // workgroups of the FP64 pass of the mixed form: 256 threads per CU
 constexpr int FP64_SCAN_GRID = 256 * 256 / (FP64_TX * FP64_TY * FP64_TZ);
 
@@ -1067,20 +1143,9 @@ struct GlobalCell {
     }
 };
 
-__global__ void constraints_kernel(BlockDesc b, const double* __restrict__ u, double* ham, double* mom,
-                                   Stencil<double> K) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.stride) return;
-    const int i = (int)(t % b.nt[0]);
-    const int j = (int)((t / b.nt[0]) % b.nt[1]);
-    const int k = (int)(t / ((long)b.nt[0] * b.nt[1]));
-    const bool inner = i >= b.ng && i < b.ng + b.n[0] && j >= b.ng && j < b.ng + b.n[1] && k >= b.ng &&
-        k < b.ng + b.n[2];
-    if (!inner) {
-        ham[t] = 0.0;
-        if (mom) mom[3 * t] = mom[3 * t + 1] = mom[3 * t + 2] = 0.0;
-        return;
-    }
+// H at interior cell t of the block and, when mom is not null, M_i into mom[0..2]
+__device__ __forceinline__ double constraint_cell(const BlockDesc& b, const double* __restrict__ u, long t,
+                                                  const Stencil<double>& K, double* mom) {
     GlobalCell C;
     C.u = u;
     C.stride = b.stride;
@@ -1169,8 +1234,8 @@ __global__ void constraints_kernel(BlockDesc b, const double* __restrict__ u, do
         Rs += ((p == 0 || p == 3 || p == 5) ? 1.0 : 2.0) * gu[p] * (R + Rc);
     }
     Rs *= chi;
-    ham[t] = Rs + (2.0 / 3.0) * Ktr * Ktr - AA;
-    if (!mom) return;
+    const double H = Rs + (2.0 / 3.0) * Ktr * Ktr - AA;
+    if (!mom) return H;
     double dAt[6][3], dK[3];
     for (int v = 0; v < 6; ++v)
         for (int d = 0; d < 3; ++d) dAt[v][d] = C.d1(AXX + v, d);
@@ -1186,9 +1251,28 @@ __global__ void constraints_kernel(BlockDesc b, const double* __restrict__ u, do
                 M -= (3.0 / (2.0 * chi + 1e-30)) * At[sym(a, c)] * gcq * dchi[q];
             }
         M -= (2.0 / 3.0) * dK[a];
-        mom[3 * t + a] = M;
+        mom[a] = M;
     }
+    return H;
 }
+
+__global__ void constraints_kernel(BlockDesc b, const double* __restrict__ u, double* ham, double* mom,
+                                   Stencil<double> K) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.stride) return;
+    const int i = (int)(t % b.nt[0]);
+    const int j = (int)((t / b.nt[0]) % b.nt[1]);
+    const int k = (int)(t / ((long)b.nt[0] * b.nt[1]));
+    const bool inner = i >= b.ng && i < b.ng + b.n[0] && j >= b.ng && j < b.ng + b.n[1] && k >= b.ng &&
+        k < b.ng + b.n[2];
+    if (!inner) {
+        ham[t] = 0.0;
+        if (mom) mom[3 * t] = mom[3 * t + 1] = mom[3 * t + 2] = 0.0;
+        return;
+    }
+    ham[t] = constraint_cell(b, u, t, K, mom ? mom + 3 * t : nullptr);
+}
+
 
 // ------------------------------------------------------------------------------------
 // initial data and noise

@@ -287,3 +287,24 @@ def group_rel(a, b):
         s = max(float(np.max(np.abs(b[grp]))), 1e-300)
         scale[grp] = s
     return np.array([np.max(np.abs(a[v] - b[v])) / scale[v] for v in range(a.shape[0])])
+
+
+def ulp_spread(fn, u, trials=2, seed=0, norm=group_rel):
+    """Per-field spread of the oracle map `fn` (input state -> output state) under 1-ulp
+    relative perturbations of its input: the largest norm(fn(u * (1 +- 2^-52)), fn(u)) over
+    `trials` random sign patterns. Non-finite outputs are left out of the comparison.
+
+    An RK3 step feeds its stage states back through second-derivative stencils, so two
+    correct implementations whose stage states differ in the last bit (any change of
+    rounding order does that) differ at the end by about this spread. Where it exceeds
+    the FP64 parity bar, the bar cannot be met by any implementation that is not bitwise
+    equal to the oracle, and the spread is the tolerance instead."""
+    rng = np.random.default_rng(seed)
+    ref = fn(u.copy())
+    bad = ~np.isfinite(ref)
+    out = np.zeros(ref.shape[0])
+    for _ in range(trials):
+        up = u * (1.0 + rng.choice([-1.0, 1.0], u.shape) * 2.0**-52)
+        got = fn(up)
+        out = np.maximum(out, norm(np.where(bad | ~np.isfinite(got), 0.0, got), np.where(bad, 0.0, ref)))
+    return out

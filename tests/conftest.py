@@ -21,6 +21,15 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
+def oracle_release():
+    """The same restatement built with the reference's Release flags (-O3 -ffast-math,
+    CMakeLists.txt:71): how far the reference as shipped departs from strict IEEE."""
+    from oracle.oracle import Oracle
+
+    return Oracle(fast=True)
+
+
+@pytest.fixture(scope="session")
 def reference():
     from oracle.oracle import Reference
 

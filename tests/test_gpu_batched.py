@@ -7,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from oracle.oracle import Geom, group_rel
+from oracle.oracle import Geom, group_rel, normwise_rel, ulp_spread
 
 pytestmark = pytest.mark.gpu
 
@@ -163,10 +163,19 @@ def test_c4_layout_batched_equals_blocks_alone(engine, precision):
             g.array = None  # release the host copy
 
 
-def test_c4_layout_against_oracle(engine, oracle):
-    """Four blocks of the batched-AMR workload (the smallest of each level) after one batched
-    RK3 step of their level, against the oracle's step of each block: <= 1e-12 of each tensor
-    group."""
+def test_c4_layout_against_oracle(engine, oracle, oracle_release):
+    """The smallest block of each level of the batched-AMR workload, in its level's batched
+    layout, against the oracle.
+
+    (a) The RHS of the initial state, per field within max(1e-12, 2 x the departure of the
+    reference's Release build from the strict oracle on the same input). Far from the
+    punctures the RHS of K, Theta and A~ is a small remainder of O(1) stencil sums (K's is
+    ~2 |grad psi|^2 / psi^8), so any change of summation order moves it: the Release build
+    itself departs by up to 4.7e-9 (K) there, 3e-14 near the punctures.
+    (b) One batched RK3 step of the level, each field within max(1e-12, 2 x spread), where
+    spread is the oracle's own change under 1-ulp perturbations of its input
+    (oracle.ulp_spread): A~ after one step is small beside the metric's last-bit noise that
+    the second-derivative stencils feed into it (spread up to 1.7e-10 of its size)."""
     import granite_amd as ga
 
     from granite_amd import workloads
@@ -176,15 +185,27 @@ def test_c4_layout_against_oracle(engine, oracle):
     for li, (dx, grids) in enumerate(c4_levels(ga, engine)):
         lvl = ga.Level(grids, engine)
         lvl.init_brill_lindquist(masses, pos)
-        say(f"C4 level {li} vs oracle")
-        lvl.rk3_step(p, 0.25 * dx, pre_fill=ga.BC_OUTFLOW)
         i = min(range(len(grids)), key=lambda q: int(np.prod(grids[q].numCells())))
-        lvl.download(i)
         b = grids[i]
         gg = Geom(b.numCells(), 4, b.lowerCorner(), b.upperCorner())
         u = oracle.init_bl(gg, masses, pos)
-        oracle.fill_ghosts(gg, u, 2)
-        oracle.rk3_step(gg, u, 0.25 * dx, 0)
-        e = group_rel(b.array, u)
-        assert e.max() < 1e-12, (li, e)
+        I = gg.interior()
+        say(f"C4 level {li}: RHS of block {i} vs oracle")
+        ref = oracle.rhs(gg, u)[I]
+        release = normwise_rel(oracle_release.rhs(gg, u)[I], ref)
+        e = normwise_rel(lvl.rhs(i, p)[I], ref)
+        assert (e <= np.maximum(1e-12, 2.0 * release)).all(), (li, "rhs", e, release)
+        say(f"C4 level {li}: RK3 step vs oracle")
+        lvl.rk3_step(p, 0.25 * dx, pre_fill=ga.BC_OUTFLOW)
+        lvl.download(i)
+
+        def step(x):
+            oracle.fill_ghosts(gg, x, 2)
+            return oracle.rk3_step(gg, x, 0.25 * dx, 0)
+
+        spread = ulp_spread(step, u)
+        ref = step(u.copy())
+        e = group_rel(b.array, ref)
+        tol = np.maximum(1e-12, 2.0 * spread)
+        assert (e <= tol).all(), (li, e, spread)
         lvl.close()

@@ -24,11 +24,10 @@ def say(msg):
 
 
 def device_sources(g, seed):
-    import torch
+    from hipmem import DeviceArray
 
     host = sources(g, seed)
-    dev = [torch.from_numpy(a.copy()).to("cuda:0") for a in host]
-    torch.cuda.synchronize()
+    dev = [DeviceArray(a) for a in host]
     return host, dev
 
 
