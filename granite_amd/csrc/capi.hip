@@ -362,6 +362,9 @@ static hipError_t build_batch(gr_level* L, const std::vector<int>& which, std::v
     B->ntiles = ntiles;
     B->ghost_wgs = (int)ghost_wg_block.size();
     for (int d = 0; d < 3; ++d) B->dx[d] = table[0].b.dx[d];
+    B->pairs_ok = 1;
+    for (const gr::BatchBlock& T : table)
+        for (int k = 0; k < 3; ++k) B->pairs_ok = B->pairs_ok && gr::pairs_aligned(T.b, T.buf[k]);
     return st;
 }
 

@@ -53,6 +53,11 @@
 #ifndef GR_FP64_LAYOUT
 #define GR_FP64_LAYOUT 0
 #endif
+// 1 = FP64 halo staged by direct global -> LDS loads (global_load_lds_dwordx4, layout 1
+// only): no registers hold the prefetched field
+#ifndef GR_FP64_DMA
+#define GR_FP64_DMA 0
+#endif
 #ifndef GR_MIXED_PREFETCH
 #define GR_MIXED_PREFETCH 1
 #endif
@@ -176,11 +181,12 @@ constexpr int conflict_free_pitch(int bx0, int by, int esize) {
 }
 
 // Lane -> cell maps of a tile's threads. Layout 0: x fastest, then y, then z. Layout 1
-// (TX = 8, TY even, TZ a multiple of 4): lane bits 0-2 are x, bit 3 the low bit of y, bits
-// 4-5 the low bits of z, and the wave index supplies the rest of y, then of z. A 64-bit LDS
-// read is served as two passes of 32 lanes; under layout 1 each pass holds 8 x-neighbours of
-// 2 rows and 2 planes, so unpadded rows (pitch TX + 8 = 16 doubles = 32 banks) stay
-// conflict-free once a plane spans 16 banks modulo 64 (a pad of 8 doubles per plane).
+// (TX = 8, TY even, TZ a multiple of 4): lane bits 0-2 are x, bit 3 the low bit of z, bit 4
+// the low bit of y, bit 5 the second bit of z, and the wave index supplies the rest of y,
+// then of z. Unpadded rows (pitch TX + 8 = 16 doubles) then read conflict-free when a
+// plane spans 8 doubles modulo 16 (a pad of 8 doubles per plane of 192): the 16 lanes of a
+// 128-byte group of a 64-bit read are 8 x-neighbours in 2 planes (banks 0-15 and 16-31 of 32),
+// and the 32 lanes of a 256-byte group add 2 rows (banks 32-63 of 64).
 template <int TX, int TY, int TZ, int LAYOUT>
 struct LaneMap {
     static_assert(LAYOUT == 0 || (TX == 8 && TY % 2 == 0 && TZ % 4 == 0), "layout 1 needs 8 x even x 4k parts");
@@ -192,27 +198,31 @@ struct LaneMap {
         } else {
             const int w = tid >> 6;
             x = tid & 7;
-            y = ((tid >> 3) & 1) + 2 * (w % (TY / 2));
-            z = ((tid >> 4) & 3) + 4 * (w / (TY / 2));
+            z = ((tid >> 3) & 1) + 2 * ((tid >> 5) & 1) + 4 * (w / (TY / 2));
+            y = ((tid >> 4) & 1) + 2 * (w % (TY / 2));
         }
     }
 };
 
-// whether every 32-lane (64-bit elements) or 64-lane (32-bit) pass of every wave reads its
-// cells from distinct banks at row pitch P and plane stride Q (elements of esize bytes)
+// whether a wave's read of its cells (elements of esize bytes, row pitch P, plane stride Q)
+// is free of bank conflicts, modelled both as 16-lane groups of 32 four-byte banks and as
+// 32-lane groups of 64 banks for 8-byte elements (64-lane groups of 64 banks for 4-byte ones)
 template <int TX, int TY, int TZ, int LAYOUT>
 constexpr bool lds_conflict_free(int P, int Q, int esize) {
-    const int words = esize / 4, lanes = 64 / words, NT = TX * TY * TZ;
-    for (int base = 0; base < NT; base += lanes) {
-        bool used[64] = {};
-        for (int t = base; t < base + lanes && t < NT; ++t) {
-            int x = 0, y = 0, z = 0;
-            LaneMap<TX, TY, TZ, LAYOUT>::cell(t, x, y, z);
-            const int a = z * Q + y * P + x;
-            for (int w = 0; w < words; ++w) {
-                const int bank = (a * words + w) % 64;
-                if (used[bank]) return false;
-                used[bank] = true;
+    const int words = esize / 4, NT = TX * TY * TZ;
+    for (int banks = 32; banks <= 64; banks *= 2) {
+        const int lanes = banks / words;
+        for (int base = 0; base < NT; base += lanes) {
+            bool used[64] = {};
+            for (int t = base; t < base + lanes && t < NT; ++t) {
+                int x = 0, y = 0, z = 0;
+                LaneMap<TX, TY, TZ, LAYOUT>::cell(t, x, y, z);
+                const int a = z * Q + y * P + x;
+                for (int w = 0; w < words; ++w) {
+                    const int bank = (a * words + w) % banks;
+                    if (used[bank]) return false;
+                    used[bank] = true;
+                }
             }
         }
     }
@@ -357,6 +367,126 @@ struct TileIO {
         __syncthreads(); // field pos is in its buffer; every thread is done with field pos-1
         stage_to_lds(pos + 1);
         prefetch(pos + 1 + PF);
+        return lds + (pos & 1) * LBUF + cl;
+    }
+
+    // ---- Sink
+    __device__ __forceinline__ void put(int v, double r, double ko, double uin) {
+        const double rr = ko_on ? r + ko : r;
+        if (MODE == MODE_RHS) {
+            if (active) gstore(out + v * stride, cb, rr);
+            return;
+        }
+        double val = combine_exact(cf.a1, unv[v], cf.a2, uin, cf.a3dt, rr);
+        if (v == CHI) val = floor_chi(val);
+        if (v == ALPHA) val = floor_alpha(val);
+        if (MODE == MODE_FINAL && v >= GXX && v < GXX + 6) {
+            hg[v - GXX] = val;
+            return;
+        }
+        if (MODE == MODE_FINAL && v >= AXX && v < AXX + 6) {
+            hA[v - AXX] = val;
+            return;
+        }
+        if (active) gstore(out + v * stride, cb, val);
+    }
+
+    __device__ __forceinline__ void finish() {
+        if (MODE != MODE_FINAL) return;
+        algebraic_exact(hg, hA);
+        if (!active) return;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            gstore(out + (GXX + a) * stride, cb, hg[a]);
+            gstore(out + (AXX + a) * stride, cb, hA[a]);
+        }
+    }
+};
+
+// TileIO whose halo staging goes straight from global memory to LDS (LDS layout 1). The
+// buffer of a field is the whole box of the part (planes of rows of TX + 8 doubles), filled
+// in 1 KiB chunks: lane l of a wave's load of chunk c writes doubles 128 c + 2 l, + 1 of the
+// buffer, fetched from the pair of x-neighbours the layout puts there (pad slots and slots
+// beyond the block fetch the field's first pair and are never read by an active cell).
+// The waves take the chunks in turn. A load's LDS data is complete once the wave's vector
+// memory counter drains, which the barrier that starts the next field waits for, so the
+// buffer of field p + 1 is loaded while field p is evaluated from the other one.
+// Needs 16-byte aligned pairs: an even row length (nt0) and 16-byte aligned buffers.
+template <int TX, int TY, int TZ, int MODE>
+struct TileIODma {
+    using T = TileGeom<TX, TY, TZ, true, 8, 1>;
+    static constexpr int CH = 128; // doubles per chunk (64 lanes x 16 bytes)
+    static constexpr int NCH = (T::NLDS + CH - 1) / CH;
+    static constexpr int LBUF = NCH * CH; // doubles per buffer
+    static constexpr int NW = T::NT / 64; // waves
+    static constexpr int KW = (NCH + NW - 1) / NW; // chunk loads per wave
+    static_assert(T::NT % 64 == 0, "whole waves");
+
+    double* lds; // 2 * LBUF doubles
+    const gdouble* in;
+    const gdouble* un;
+    gdouble* out;
+    long stride;
+    long cidx;
+    unsigned cb;
+    int cl;
+    int wave;
+    bool active;
+    bool ko_on;
+    StageCoef cf;
+    unsigned doff[KW]; // byte offset of this lane's pair in each of its wave's chunks
+    double unb;
+    double unv[NV];
+    double hg[6], hA[6];
+
+    __device__ void init(int tid, int ox, int oy, int oz, int nt0, int nt1, int nt2) {
+        wave = __builtin_amdgcn_readfirstlane(tid / 64); // uniform: LDS bases go to M0 from scalars
+        const int lane = tid % 64;
+#pragma unroll
+        for (int k = 0; k < KW; ++k) {
+            const int c = wave + k * NW;
+            const int e = c * CH + 2 * lane;
+            const int pl = e / T::PLANE, r = e % T::PLANE;
+            const int x = r % T::BX - HALO, y = r / T::BX - HALO, z = pl - HALO;
+            const int gx = ox + x, gy = oy + y, gz = oz + z;
+            const bool ok = c < NCH && pl < T::BZ && r < T::BX * T::BY && gx + 1 < nt0 && gy < nt1 && gz < nt2;
+            doff[k] = ok ? 8u * (unsigned)(gx + nt0 * (gy + nt1 * gz)) : 0u;
+        }
+        prefetch(0);
+    }
+
+    __device__ __forceinline__ void prefetch(int pos) {
+        if (pos >= NV) return;
+        const int v = FIELD_ORDER[pos];
+        const gchar* f = (const gchar*)(in + v * stride);
+        double* L = lds + (pos & 1) * LBUF;
+#pragma unroll
+        for (int k = 0; k < KW; ++k) {
+            const int c = wave + k * NW; // uniform over the wave
+            if (c < NCH) {
+                unsigned o = doff[k];
+#ifdef __HIP_DEVICE_COMPILE__
+                asm volatile("" : "+v"(o)); // keeps the 32-bit offset at the load (scalar base + offset)
+#endif
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(f + o),
+                                                 (__attribute__((address_space(3))) void*)(L + c * CH), 16, 0, 0);
+            }
+        }
+        if (MODE != MODE_RHS) unb = gload(un + v * stride, cb);
+    }
+
+    // ---- Stager
+    __device__ double centre(int v) const { return gload(in + v * stride, cb); }
+    __device__ void metric64(double g[6]) const {
+#pragma unroll
+        for (int a = 0; a < 6; ++a) g[a] = gload(in + (GXX + a) * stride, cb);
+    }
+    __device__ FixedStrides<1, T::BX, T::PLANE> strides() const { return {}; }
+    __device__ const double* field(int v) {
+        const int pos = field_pos(v);
+        __syncthreads(); // field pos has landed in its buffer; every thread is done with pos - 1
+        if (MODE != MODE_RHS) unv[v] = unb;
+        prefetch(pos + 1);
         return lds + (pos & 1) * LBUF + cl;
     }
 
@@ -633,7 +763,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
     const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
     const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
 
+#if GR_FP64_DMA
+    static_assert(GR_FP64_LAYOUT == 1, "direct LDS staging needs LDS layout 1");
+    TileIODma<TX, TY, TZ, MODE> io;
+#else
     TileIO<TX, TY, TZ, MODE, GR_FP64_PREFETCH> io;
+#endif
     io.lds = lds;
     io.in = R.in;
     io.un = R.un;
@@ -655,6 +790,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
 // parts of a tile in the FP64 kernels
 constexpr int FP64_PARTS = GR_FP64_SPLIT * GR_FP64_ZSPLIT;
 
+// LDS doubles of one FP64 part's two field buffers
+template <int TX, int TY, int TZ>
+constexpr int fp64_lds_doubles() {
+    return GR_FP64_DMA ? 2 * TileIODma<TX, TY, TZ, MODE_RHS>::LBUF
+                       : 2 * (TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>::NLDS + 1);
+}
+
 // whether part `part` of tile R holds no cell of the block (uniform over the workgroup)
 template <int TX, int TZ>
 __device__ __forceinline__ bool part_outside(const TileRef& R, int part) {
@@ -665,8 +807,7 @@ __device__ __forceinline__ bool part_outside(const TileRef& R, int part) {
 // one part of one tile per workgroup (FP64_PARTS consecutive workgroups per tile)
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
-    using T = TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>;
-    __shared__ double lds[2 * (T::NLDS + 1)];
+    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ>()];
     const int slot = tile_of_block(blockIdx.x, A.ntiles * FP64_PARTS, A.xcd_remap);
     const int tile = slot / FP64_PARTS, part = slot % FP64_PARTS;
     if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
@@ -680,11 +821,11 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(St
 // (every thread takes the same trip counts, so the barriers are uniform)
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_scan(StageArgs A) {
-    using T = TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>;
-    __shared__ double lds[2 * (T::NLDS + 1)];
-    __shared__ int list[T::NT];
+    constexpr int NT = TX * TY * TZ;
+    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ>()];
+    __shared__ int list[NT];
     __shared__ int nlist;
-    const int per = (int)gridDim.x * T::NT;
+    const int per = (int)gridDim.x * NT;
     for (int base = 0; base < A.ntiles; base += per) {
         if (threadIdx.x == 0) nlist = 0;
         __syncthreads();
@@ -1604,8 +1745,13 @@ int batch_layout(std::vector<BatchBlock>& blocks, std::vector<int>& tile_block, 
     return t;
 }
 
+bool pairs_aligned(const BlockDesc& b, const void* p) {
+    return b.nt[0] % 2 == 0 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
 Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src) {
     Batch B;
+    B.pairs_ok = pairs_aligned(b, in) && pairs_aligned(b, un) && pairs_aligned(b, out);
     B.table = nullptr;
     B.tile_block = nullptr;
     B.ghost_wg_block = nullptr;
@@ -1629,6 +1775,8 @@ Batch single_batch(const BlockDesc& b, const double* in, const double* un, doubl
 hipError_t launch_stage_fp64(const Batch& B, const int sel[3], bool has_src, const CellParams& P,
                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags) {
+    // the direct LDS staging reads 16-byte pairs
+    if (GR_FP64_DMA && !B.pairs_ok) return hipErrorInvalidValue;
     StageArgs A = stage_args(B, sel, P, ko_sigma, cf, flags, 1);
     // with flags (the FP64 tiles of the mixed form: few, near the puncture) a small grid
     // gathers them instead of launching one workgroup per tile of the level

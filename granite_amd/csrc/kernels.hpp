@@ -60,7 +60,10 @@ struct Batch {
     int ntiles;
     int ghost_wgs; // workgroups of a ghost launch
     double dx[3]; // grid spacing shared by all blocks of the launch
+    int pairs_ok; // every block: even row length (nt0) and 16-byte aligned buffers
 };
+// whether block b's rows can be read as 16-byte pairs from buffer p
+bool pairs_aligned(const BlockDesc& b, const void* p);
 
 // builds the tile bookkeeping of a block list (host side): tile0/tiles and ghost_wg0 of
 // each block, the tile->block map and the ghost-workgroup->block map. Returns the total
