@@ -112,6 +112,7 @@ def library():
         "gr_level_rk3_step": (C.c_int, [vp, C.POINTER(_Params), C.c_double, C.c_int, C.c_int, C.c_int, C.c_int]),
         "gr_level_sync": (C.c_int, [vp]),
         "gr_level_constraints_l2": (C.c_int, [vp, _dp, C.POINTER(C.c_longlong)]),
+        "gr_ccz4_constraints_l2_noalloc": (C.c_int, [vp, C.POINTER(_Geom), vp, vp, C.POINTER(C.c_longlong)]),
         "gr_level_nonfinite": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
         "gr_level_read_cell": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
         "gr_level_profile": (C.c_int, [vp, C.c_int]),

@@ -18,13 +18,24 @@
 #define GR_MIXED_OVERLAP 1
 #endif
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
 
+// Reduction partials of the allocation-free constraint monitor
+// (gr_ccz4_constraints_l2_noalloc): a file-scope device array, one copy per GPU, held until
+// the process ends. In host code `scratch` names a host-side stand-in, never device memory:
+// its device address is resolved per device with hipGetSymbolAddress when an engine is
+// created on that device (noalloc_scratch below) and only that address is passed to kernels.
+// Every call on one GPU shares the array; calls that overlap on one GPU race.
+constexpr int kNoAllocPartials = 2048;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
 struct gr_engine {
     int device = -1;
     hipStream_t stream = nullptr;
+    double* scratch = nullptr; // device address of `scratch` on this engine's device
     gr::AuxStream aux{nullptr, nullptr, nullptr}; // runs the FP64 tiles beside the FP32 ones
     std::string err;
     // compatibility-mode staging buffers (grown on demand, doubles)
@@ -157,6 +168,23 @@ int use_device(gr_engine* e) {
     return GR_OK;
 }
 
+// Device address of `scratch` on `device` (which must be current), looked up once per device
+// and kept for the life of the process.
+hipError_t noalloc_scratch(int device, double** out) {
+    static std::mutex mu;
+    static std::vector<double*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (st != hipSuccess) return st;
+        addr[device] = static_cast<double*>(p);
+    }
+    *out = addr[device];
+    return hipSuccess;
+}
+
 } // namespace
 
 extern "C" {
@@ -184,6 +212,10 @@ int gr_engine_create(int device, gr_engine** out) {
     e->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess) {
         delete e;
+        return GR_ERR_NODEV;
+    }
+    if (noalloc_scratch(device, &e->scratch) != hipSuccess) {
+        gr_engine_destroy(e);
         return GR_ERR_NODEV;
     }
 #if GR_MIXED_OVERLAP
@@ -791,6 +823,25 @@ int gr_level_constraints_l2(gr_level* L, double* sumsq, long long* count) {
     }
     *sumsq = total;
     *count = n;
+    return GR_OK;
+}
+
+int gr_ccz4_constraints_l2_noalloc(gr_engine* e, const gr_geom* geom, const double* d_u, double* d_sumsq,
+                                   long long* count) {
+    int rc = use_device(e);
+    if (rc) return rc;
+    gr::BlockDesc b;
+    if ((rc = check_geom(e, geom, &b))) return rc;
+    if (!d_u || !d_sumsq || !count) return fail(e, GR_ERR_ARG, "null state or output");
+    if (!e->scratch) return fail(e, GR_ERR_NODEV, "no scratch address on this device");
+    // the partition gives each workgroup at least ceil(cells / kNoAllocPartials) cells, so
+    // every block the geometry check accepts fits the scratch; checked before any launch
+    if (gr::reduce_parts(b, kNoAllocPartials) > kNoAllocPartials)
+        return fail(e, GR_ERR_GEOM, "block needs more reduction partials than the scratch holds (2048)");
+    int nparts = 0;
+    GR_HIP(e, gr::launch_ham_sumsq(b, d_u, e->scratch, kNoAllocPartials, &nparts, e->stream));
+    GR_HIP(e, gr::launch_sum_partials(e->scratch, nparts, d_sumsq, e->stream));
+    *count = (long long)b.n[0] * b.n[1] * b.n[2];
     return GR_OK;
 }
 

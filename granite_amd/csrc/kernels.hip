@@ -1515,6 +1515,27 @@ __global__ __launch_bounds__(RED_T) void sumsq_kernel(BlockDesc b, const double*
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// sum over the interior of H^2 with H formed on the fly (no per-cell H array): the partition
+// and the visiting order of sumsq_kernel over constraints_kernel's output
+__global__ __launch_bounds__(RED_T) void ham_sumsq_kernel(BlockDesc b, const double* __restrict__ u,
+                                                          Stencil<double> K, double* partial, long per) {
+    __shared__ double sh[RED_T];
+    const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
+    const long beg = (long)blockIdx.x * per;
+    const long end = beg + per < ncell ? beg + per : ncell;
+    double acc = 0.0;
+    for (long t = beg + threadIdx.x; t < end; t += RED_T) {
+        const int i = b.ng + (int)(t % b.n[0]);
+        const int j = b.ng + (int)((t / b.n[0]) % b.n[1]);
+        const int k = b.ng + (int)(t / ((long)b.n[0] * b.n[1]));
+        const long c = i + (long)b.nt[0] * (j + (long)b.nt[1] * k);
+        const double h = constraint_cell(b, u, c, K, nullptr);
+        acc += h * h;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(RED_T) void nonfinite_kernel(BlockDesc b, const double* u, double* partial, long per) {
     __shared__ double sh[RED_T];
     const long ncell = (long)b.n[0] * b.n[1] * b.n[2];
@@ -1906,6 +1927,19 @@ hipError_t launch_sumsq_interior(const BlockDesc& b, const double* ham, int ncom
                                  int* nparts, hipStream_t s) {
     const long per = reduce_split(b, cap, nparts);
     sumsq_kernel<<<*nparts, RED_T, 0, s>>>(b, ham, ncomp, partial, per);
+    return hipGetLastError();
+}
+
+int reduce_parts(const BlockDesc& b, int cap) {
+    int n = 0;
+    (void)reduce_split(b, cap, &n);
+    return n;
+}
+
+hipError_t launch_ham_sumsq(const BlockDesc& b, const double* u, double* partial, int cap, int* nparts,
+                            hipStream_t s) {
+    const long per = reduce_split(b, cap, nparts);
+    ham_sumsq_kernel<<<*nparts, RED_T, 0, s>>>(b, u, make_stencil(b, 0.0), partial, per);
     return hipGetLastError();
 }
 

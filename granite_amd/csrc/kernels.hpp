@@ -149,6 +149,11 @@ hipError_t launch_sumsq_interior(const BlockDesc& b, const double* ham, int ncom
                                  int* nparts, hipStream_t s);
 // sums `n` doubles into out[0] with one workgroup (fixed order)
 hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
+// number of partials the interior reductions split a block into (at most `cap`)
+int reduce_parts(const BlockDesc& b, int cap);
+// partial sums of H^2 over the interior (H formed on the fly), at most `cap` partials
+hipError_t launch_ham_sumsq(const BlockDesc& b, const double* u, double* partial, int cap, int* nparts,
+                            hipStream_t s);
 // partial[blockIdx] = number of non-finite interior values over all 22 fields
 hipError_t launch_count_nonfinite(const BlockDesc& b, const double* u, double* partial, int cap, int* nparts,
                                   hipStream_t s);

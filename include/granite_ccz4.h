@@ -187,6 +187,17 @@ int gr_level_sync(gr_level* lvl);
  * FP64 monitor (the reference prints sqrt(sumsq/count), src/main.cpp:1216-1250). */
 int gr_level_constraints_l2(gr_level* lvl, double* sumsq, long long* count);
 
+/* Allocation-free constraint monitor: sum of H^2 over the interior of one block whose state
+ * the caller holds in device memory (GridBlock layout), written to the device double
+ * *d_sumsq; *count = interior cells. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops). It
+ * calls no hipMalloc/hipFree: the reduction partials live in a fixed device array of 2048
+ * doubles per GPU, shared by every call on that GPU (calls that overlap on one GPU race;
+ * each call rewrites what it reads). Enqueued on the engine stream, not synchronised
+ * (gr_engine_sync). Blocks over 2^29 cells with ghosts are rejected (GR_ERR_GEOM). */
+int gr_ccz4_constraints_l2_noalloc(gr_engine* eng, const gr_geom* geom, const double* d_u, double* d_sumsq,
+                                   long long* count);
+
 /* Number of non-finite values in the interiors of all blocks' state (NaN scan of
  * src/main.cpp:1192-1214). */
 int gr_level_nonfinite(gr_level* lvl, long long* count);
