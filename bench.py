@@ -106,6 +106,20 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "copies": 5}), flush=True)
 
 
+STAGING = {0: "registers -> padded LDS", 1: "direct global -> LDS"}
+
+
+def level_tuning(ga, lvl, times=None):
+    """The autotuner's choices in force on a level (and the per-variant times it measured)."""
+    ngroups = len(times["stage"]) if times else 1
+    out = {"stage_staging": [STAGING[lvl.tuning(ga.TUNE_STAGE, g)] for g in range(ngroups)],
+           "ghost_field_groups": lvl.tuning(ga.TUNE_GHOST, 0)}
+    if times:
+        out["stage_ms_per_variant"] = times["stage"]
+        out["ghost_ms_per_groups"] = {str(k): v for k, v in times["ghost"].items()}
+    return out
+
+
 def run_batched(ga, eng, p, args):
     """Uneven AMR blocks (granite_amd.workloads.uneven_amr_levels: ~250 blocks, edges in
     {16..64}, 4 levels): one RK3 step of every level (each with dt = 0.25 dx_l), either
@@ -119,9 +133,12 @@ def run_batched(ga, eng, p, args):
     steps = max(3, args.steps // 5)
     out = {"blocks": sum(len(b) for _, b in layout), "levels": len(layout), "cells": total}
 
-    def timed(levels, precision):
+    def timed(levels, precision, tune):
+        tuning = []
         for lv, dx in levels:
             lv.init_brill_lindquist(masses, pos)
+            if tune and not args.no_tune:
+                tuning.append(level_tuning(ga, lv, lv.autotune(p, 0.25 * dx)))
         for _ in range(2):
             for lv, dx in levels:
                 lv.rk3_step(p, 0.25 * dx, precision=precision, pre_fill=ga.BC_OUTFLOW, use_graph=True)
@@ -135,7 +152,10 @@ def run_batched(ga, eng, p, args):
             lv.sync()
         ms = (time.perf_counter() - t0) * 1e3 / steps
         bad = sum(lv.nonfinite() for lv, _ in levels)
-        return {"ms_per_step": ms, "value": total / (ms * 1e-3), "nonfinite": bad}
+        r = {"ms_per_step": ms, "value": total / (ms * 1e-3), "nonfinite": bad}
+        if tuning:
+            r["tuning_per_level"] = tuning
+        return r
 
     def make(batched):
         levels = []
@@ -150,7 +170,7 @@ def run_batched(ga, eng, p, args):
     for name, batched, prec in (("fp64_batched", True, ga.PREC_FP64), ("mixed_batched", True, ga.PREC_MIXED),
                                 ("fp64_per_block", False, ga.PREC_FP64)):
         levels = make(batched)
-        out[name] = timed(levels, prec)
+        out[name] = timed(levels, prec, batched)
         for lv, _ in levels:
             lv.close()
     out["batched_speedup_vs_per_block"] = out["fp64_per_block"]["ms_per_step"] / out["fp64_batched"]["ms_per_step"]
@@ -159,7 +179,7 @@ def run_batched(ga, eng, p, args):
     return out
 
 
-def run_noisy_192(ga, eng, p, steps):
+def run_noisy_192(ga, eng, p, steps, tune=True):
     """The mixed-precision single-puncture case: BL M = 1, one 192^3 block on [-16,16]^3
     (dx = 1/6, dt = 1/24) with N(0, 1e-8^2) added to the conformal metric and K (seed
     20240611, the engine's portable generator). Per-field max RHS error of the mixed form
@@ -178,6 +198,8 @@ def run_noisy_192(ga, eng, p, steps):
         lvl = ga.Level([blk], eng)
         lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
         lvl.add_noise(20240611, 1e-8, noise_fields)
+        if tune:
+            out.setdefault("tuning", level_tuning(ga, lvl, lvl.autotune(p, dt)))
         r = lvl.rhs(0, p, prec)[:, 4:-4, 4:-4, 4:-4]
         res[name] = r
         st = dict(precision=prec, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
@@ -231,7 +253,7 @@ def checkpoint_staging(ga, eng, levels, H, p, dt0, prec, ms_step):
             "coarse_step_ms_during_transfer": (t3 - t2) * 1e3 / 3, "coarse_step_ms_alone": ms_step}
 
 
-def run_hierarchy(ga, eng, p):
+def run_hierarchy(ga, eng, p, args):
     """The 4-level binary-puncture hierarchy (granite_amd.hierarchy.binary_puncture_levels:
     96^3 on [-64,64]^3 plus tracking-sphere levels of 106x96^2, 114x96^2, 132x96^2 cells at
     dx 2/3, 1/3, 1/6; punctures m = 0.5 at (+-3,0,0)) evolved to t = 10 M: 30 Berger-Oliger
@@ -252,6 +274,11 @@ def run_hierarchy(ga, eng, p):
         levels[0].init_brill_lindquist(masses, pos)
         for l in range(1, len(levels)):
             H.prolongate(l)
+        tuning = None
+        if not args.no_tune:
+            H.autotune(p, dt0)
+            tuning = {"per_level": [level_tuning(ga, lv) for lv in levels],
+                      "transfer_field_groups": [H.tuning(l) for l in range(1, len(levels))]}
         H.fill_ghosts(0)
         H.step(p, dt0, prec, use_graph=True)
         H.sync()
@@ -264,6 +291,8 @@ def run_hierarchy(ga, eng, p):
         bad = sum(lv.nonfinite() for lv in levels)
         out[name] = {"ms_per_coarse_step": ms, "value": per_step / (ms * 1e-3), "ham_l2_t10": (sh / n) ** 0.5,
                      "mom_l2_t10": (sm / n) ** 0.5, "nonfinite": bad}
+        if tuning:
+            out[name]["tuning"] = tuning
         if name == "fp64":
             out["checkpoint_staging"] = checkpoint_staging(ga, eng, levels, H, p, dt0, prec, ms)
         H.close()
@@ -290,6 +319,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-AMR-blocks section")
     ap.add_argument("--no-hierarchy", action="store_true", help="skip the binary-puncture hierarchy section")
     ap.add_argument("--no-noisy", action="store_true", help="skip the 192^3 noisy mixed-precision section")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default kernel variants (no autotuning)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -311,6 +341,7 @@ def main():
         blk = ga.GridBlock(0, 0, (N, N, N), (-HALF,) * 3, (HALF,) * 3, 4)
         lvl = ga.Level([blk], eng)
         lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
+        tuning = None if args.no_tune else level_tuning(ga, lvl, lvl.autotune(p, DT))
         step = dict(precision=precision, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
         for _ in range(args.warmup):
             lvl.rk3_step(p, DT, use_graph=True, **step)
@@ -338,6 +369,7 @@ def main():
             "final_launches": prof[1][1],
             "ghost_avg_ms": prof[2][0] / max(prof[2][1], 1),
             "nonfinite": lvl.nonfinite(),
+            "tuning": tuning,
             "t": (args.warmup + args.steps + args.profile_steps) * DT,
         }
         sumsq, count = lvl.constraints_l2()
@@ -348,8 +380,8 @@ def main():
     f64 = run_form(ga.PREC_FP64)
     mix = run_form(ga.PREC_MIXED)
     batched = None if args.no_batched else run_batched(ga, eng, p, args)
-    hier = None if args.no_hierarchy else run_hierarchy(ga, eng, p)
-    noisy = None if args.no_noisy else run_noisy_192(ga, eng, p, 100)
+    hier = None if args.no_hierarchy else run_hierarchy(ga, eng, p, args)
+    noisy = None if args.no_noisy else run_noisy_192(ga, eng, p, 100, tune=not args.no_tune)
     value = f64["value"]
     ms_per_step = f64["ms_per_step"]
 
@@ -413,6 +445,7 @@ def main():
             "dt": DT,
             "precision": "fp64",
             "stage_tile": list(_tile(ga)),
+            "tuning": f64.get("tuning"),
             "graph": True,
         },
         "roofline": {

@@ -14,6 +14,9 @@ from .ccz4 import (  # noqa: F401
     BC_NONE,
     PREC_FP64,
     PREC_MIXED,
+    TUNE_GHOST,
+    TUNE_STAGE,
+    TUNE_TRANSFER,
     CCZ4Evolution,
     CCZ4Params,
     Checkpoint,
@@ -25,4 +28,7 @@ from .ccz4 import (  # noqa: F401
     SpacetimeVar,
     library,
     library_path,
+    tune_clear,
+    tune_load,
+    tune_save,
 )

@@ -19,6 +19,9 @@ import numpy as np
 NUM_SPACETIME_VARS = 22
 BC_NONE, BC_SOMMERFELD, BC_PERIODIC, BC_OUTFLOW = -1, 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+# tuned choices (enum gr_tune_kind)
+TUNE_STAGE, TUNE_GHOST, TUNE_TRANSFER = 0, 1, 2
+GROUP_CHOICES = (1, 2, 4, 11)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -140,6 +143,15 @@ def library():
         "gr_checkpoint_wait": (C.c_int, [vp]),
         "gr_checkpoint_nblocks": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "gr_checkpoint_block": (C.c_int, [vp, C.c_int, C.POINTER(_dp), C.POINTER(C.c_long)]),
+        "gr_level_autotune": (C.c_int, [vp, C.POINTER(_Params), C.c_double, _dp, C.c_int, C.POINTER(C.c_int)]),
+        "gr_hierarchy_autotune": (C.c_int, [vp, C.POINTER(_Params), C.c_double, _dp, C.c_int, C.POINTER(C.c_int)]),
+        "gr_level_get_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+        "gr_level_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "gr_hierarchy_get_tuning": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+        "gr_hierarchy_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
+        "gr_tune_save": (C.c_int, [C.c_char_p]),
+        "gr_tune_load": (C.c_int, [C.c_char_p]),
+        "gr_tune_clear": (C.c_int, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -155,6 +167,23 @@ def _ptr(a):
     if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
         raise TypeError("expected a C-contiguous float64 numpy array")
     return a.ctypes.data_as(_dp)
+
+
+def tune_save(path):
+    """Write the process-wide tuning table (gr_tune_save)."""
+    if library().gr_tune_save(os.fsencode(path)) != 0:
+        raise EngineError(-1, f"cannot write tuning table {path}")
+
+
+def tune_load(path):
+    """Merge a tuning table file into the process-wide table (gr_tune_load); levels created
+    afterwards use its entries."""
+    if library().gr_tune_load(os.fsencode(path)) != 0:
+        raise EngineError(-1, f"cannot read tuning table {path} (missing, or a malformed / unknown entry)")
+
+
+def tune_clear():
+    library().gr_tune_clear()
 
 
 # ----------------------------------------------------------------------------------------
@@ -445,6 +474,27 @@ class Level:
     def set_mixed_policy(self, chi_fp64_below):
         self._e.check(self._e.lib.gr_level_set_mixed_policy(self._h, float(chi_fp64_below)))
 
+    def autotune(self, params: CCZ4Params, dt):
+        """Time the stage-kernel staging variants (per group of equal spacing) and the ghost
+        fill's field groups on this level, keep the fastest (gr_level_autotune). Returns
+        {"stage": [[ms per variant, None where not applicable] per group], "ghost": {groups: ms}}."""
+        cap = 256
+        t = (C.c_double * cap)()
+        n = C.c_int()
+        self._e.check(self._e.lib.gr_level_autotune(self._h, C.byref(params.c()), float(dt), t, cap, C.byref(n)))
+        vals = [t[i] for i in range(n.value)]
+        ng = len(vals) - len(GROUP_CHOICES)
+        stage = [[None if v < 0 else v for v in vals[i:i + 2]] for i in range(0, ng, 2)]
+        return {"stage": stage, "ghost": dict(zip(GROUP_CHOICES, vals[ng:]))}
+
+    def tuning(self, kind=TUNE_STAGE, group=0):
+        v = C.c_int()
+        self._e.check(self._e.lib.gr_level_get_tuning(self._h, int(kind), int(group), C.byref(v)))
+        return v.value
+
+    def set_tuning(self, kind, group, value):
+        self._e.check(self._e.lib.gr_level_set_tuning(self._h, int(kind), int(group), int(value)))
+
     def rhs(self, i, params: CCZ4Params, precision=PREC_FP64):
         """RHS of block i's device state (host array shaped like the block)."""
         out = np.zeros_like(self.blocks[i].array)
@@ -511,6 +561,23 @@ class Hierarchy:
 
     def set_cfl_monitor(self, enable=True):
         self._e.check(self._e.lib.gr_hierarchy_set_cfl_monitor(self._h, 1 if enable else 0))
+
+    def autotune(self, params: CCZ4Params, dt0):
+        """gr_hierarchy_autotune: each level's variants (dt0 / 2^l), then the transfer field
+        groups of every level l > 0. Returns the raw per-launch times (ms) in the C ABI's order."""
+        cap = 1024
+        t = (C.c_double * cap)()
+        n = C.c_int()
+        self._e.check(self._e.lib.gr_hierarchy_autotune(self._h, C.byref(params.c()), float(dt0), t, cap, C.byref(n)))
+        return [t[i] for i in range(n.value)]
+
+    def tuning(self, level):
+        v = C.c_int()
+        self._e.check(self._e.lib.gr_hierarchy_get_tuning(self._h, int(level), C.byref(v)))
+        return v.value
+
+    def set_tuning(self, level, groups):
+        self._e.check(self._e.lib.gr_hierarchy_set_tuning(self._h, int(level), int(groups)))
 
     def cfl(self):
         """[(level, advective CFL)] of the last coarse step, one per level step in subcycle order."""

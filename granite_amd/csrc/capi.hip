@@ -12,6 +12,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <tuple>
 
 // run the FP64 tiles of the mixed form on a second stream, beside the FP32 kernel
 #ifndef GR_MIXED_OVERLAP
@@ -357,6 +361,60 @@ int gr_ccz4_constraints(gr_engine* e, const gr_geom* geom, const double* u, doub
 
 } // extern "C"
 
+// ---------------------------------------------------------------------------------------
+// Tuning table: the kernel variant chosen for each (choice, block-size class, precision).
+// Choices: the FP64 stage kernel's staging variant (0 registers, 1 direct LDS), the ghost
+// fill's field groups (1, 2, 4, 11) and the AMR transfers' field groups (1, 2, 4, 11). The
+// class is the mean interior cells per block: < 24^3, < 48^3, < 96^3, larger. Levels and
+// hierarchies read the table when they are created; gr_level_autotune / gr_hierarchy_autotune
+// time the variants on their own blocks and write the winners; gr_tune_save / gr_tune_load
+// keep the table in a text file between processes.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tune;
+
+int size_class(double cells) {
+    if (cells < 24.0 * 24 * 24) return 0;
+    if (cells < 48.0 * 48 * 48) return 1;
+    if (cells < 96.0 * 96 * 96) return 2;
+    return 3;
+}
+
+template <class V>
+int blocks_class(const gr_level* L, const V& which) {
+    double c = 0.0;
+    for (int i : which) c += (double)L->blk[i].n[0] * L->blk[i].n[1] * L->blk[i].n[2];
+    return size_class(which.empty() ? 0.0 : c / (double)which.size());
+}
+
+int level_class(const gr_level* L) {
+    std::vector<int> all(L->blk.size());
+    for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
+    return blocks_class(L, all);
+}
+
+bool tune_get(int kind, int cls, int prec, int* v) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(std::make_tuple(kind, cls, prec));
+    if (it == g_tune.end()) return false;
+    *v = it->second;
+    return true;
+}
+
+void tune_put(int kind, int cls, int prec, int v) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[std::make_tuple(kind, cls, prec)] = v;
+}
+
+bool tune_valid(int kind, int v) {
+    if (kind == GR_TUNE_STAGE) return v >= 0 && v < gr::FP64_STAGE_VARIANTS;
+    return gr::ghost_groups_valid(v); // GR_TUNE_GHOST, GR_TUNE_TRANSFER
+}
+
+} // namespace
+
 // The batched launch description of blocks `which` of a level: host table (geometry, the
 // three buffers, sources), its device copy, the tile -> block and ghost workgroup -> block
 // maps; one block needs no table.
@@ -453,6 +511,12 @@ int gr_level_create(gr_engine* e, int nblocks, const gr_geom* geoms, gr_level** 
         if (st != hipSuccess) break;
         st = build_batch(L, G.blocks, G.table, &G.d_table, &G.d_tile_block, &G.d_ghost_wg_block, &G.batch);
         if (st == hipSuccess) st = hipMalloc((void**)&G.flags, G.batch.ntiles);
+        int v = 0;
+        if (tune_get(GR_TUNE_STAGE, blocks_class(L, G.blocks), GR_PREC_FP64, &v)) G.batch.variant = v;
+    }
+    {
+        int g = 0;
+        if (tune_get(GR_TUNE_GHOST, level_class(L), GR_PREC_FP64, &g)) L->batch.ghost_groups = g;
     }
     L->maxstride = maxstride;
     if (st == hipSuccess) st = hipMalloc((void**)&L->ham, maxstride * sizeof(double));
@@ -761,11 +825,14 @@ int gr_level_rhs(gr_level* L, int b, const gr_ccz4_params* prm, int precision, d
     GR_HIP(e, hipMemsetAsync(L->s1[b], 0, n * sizeof(double), e->stream));
     static const int sel[3] = {0, 1, 2};
     unsigned char* flags = nullptr;
+    gr::Batch B = gr::single_batch(L->blk[b], L->u[b], L->u[b], L->s1[b], L->src[b]);
     for (gr_group& G : L->groups)
         for (int i : G.blocks)
-            if (i == b) flags = G.flags;
-    GR_HIP(e, stage_batch(L, gr::single_batch(L->blk[b], L->u[b], L->u[b], L->s1[b], L->src[b]), flags, sel, prm,
-                          gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0}, precision));
+            if (i == b) {
+                flags = G.flags;
+                B.variant = G.batch.variant; // the staging variant of the block's group
+            }
+    GR_HIP(e, stage_batch(L, B, flags, sel, prm, gr::MODE_RHS, gr::StageCoef{0.0, 0.0, 0.0}, precision));
     GR_HIP(e, hipMemcpyAsync(host_rhs, L->s1[b], n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     GR_HIP(e, hipStreamSynchronize(e->stream));
     return GR_OK;
@@ -964,6 +1031,170 @@ int gr_level_cfl(gr_level* L, double dt, double* max_cfl) {
 
 } // extern "C"
 
+namespace {
+
+// mean device time (ms) of `reps` launches on the engine stream, after one untimed launch
+template <class F>
+hipError_t time_launches(gr_engine* e, int reps, F&& launch, double* ms) {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t st = hipEventCreate(&a);
+    if (st == hipSuccess) st = hipEventCreate(&b);
+    if (st == hipSuccess) st = launch();
+    if (st == hipSuccess) st = hipEventRecord(a, e->stream);
+    for (int r = 0; r < reps && st == hipSuccess; ++r) st = launch();
+    if (st == hipSuccess) st = hipEventRecord(b, e->stream);
+    if (st == hipSuccess) st = hipEventSynchronize(b);
+    float t = 0.0f;
+    if (st == hipSuccess) st = hipEventElapsedTime(&t, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    *ms = (double)t / reps;
+    return st;
+}
+
+constexpr int kGroupChoices[4] = {1, 2, 4, 11};
+
+} // namespace
+
+extern "C" {
+
+int gr_level_autotune(gr_level* L, const gr_ccz4_params* prm, double dt, double* times, int cap, int* n) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if ((rc = check_params(e, prm))) return rc;
+    if (!std::isfinite(dt)) return fail(e, GR_ERR_ARG, "dt must be finite");
+    if (cap > 0 && !times) return fail(e, GR_ERR_ARG, "null times");
+    std::vector<double> out;
+    // stage 1 reads the state and writes the first stage buffer only
+    static const int sel1[3] = {0, 0, 1};
+    const gr::StageCoef c1 = stage_coef(1, dt);
+    for (gr_group& G : L->groups) {
+        int best = 0;
+        double tbest = INFINITY;
+        for (int v = 0; v < gr::FP64_STAGE_VARIANTS; ++v) {
+            if (v == 1 && !G.batch.pairs_ok) {
+                out.push_back(-1.0);
+                continue;
+            }
+            gr::Batch B = G.batch;
+            B.variant = v;
+            double ms = 0.0;
+            GR_HIP(e, time_launches(
+                          e, 3, [&] { return stage_batch(L, B, nullptr, sel1, prm, gr::MODE_STAGE, c1, GR_PREC_FP64); },
+                          &ms));
+            out.push_back(ms);
+            if (ms < tbest) {
+                tbest = ms;
+                best = v;
+            }
+        }
+        G.batch.variant = best;
+        tune_put(GR_TUNE_STAGE, blocks_class(L, G.blocks), GR_PREC_FP64, best);
+    }
+    int gbest = L->batch.ghost_groups;
+    double tbest = INFINITY;
+    for (int g : kGroupChoices) {
+        gr::Batch B = L->batch;
+        B.ghost_groups = g;
+        double ms = 0.0;
+        // the outflow fill of the first stage buffer's ghosts
+        GR_HIP(e, time_launches(e, 5, [&] { return gr::launch_fill_ghosts_batch(B, 1, GR_BC_OUTFLOW, e->stream); },
+                                &ms));
+        out.push_back(ms);
+        if (ms < tbest) {
+            tbest = ms;
+            gbest = g;
+        }
+    }
+    L->batch.ghost_groups = gbest;
+    tune_put(GR_TUNE_GHOST, level_class(L), GR_PREC_FP64, gbest);
+    ++L->generation; // captured steps hold the launch forms
+    for (int i = 0; i < cap && i < (int)out.size(); ++i) times[i] = out[i];
+    if (n) *n = (int)std::min<size_t>(out.size(), (size_t)std::max(cap, 0));
+    return GR_OK;
+}
+
+int gr_level_get_tuning(gr_level* L, int kind, int group, int* value) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    if (!value) return fail(e, GR_ERR_ARG, "null output");
+    if (kind == GR_TUNE_STAGE) {
+        if (group < 0 || group >= (int)L->groups.size()) return fail(e, GR_ERR_ARG, "group index out of range");
+        const gr::Batch& B = L->groups[group].batch;
+        *value = (B.variant == 1 && B.pairs_ok) ? 1 : 0;
+        return GR_OK;
+    }
+    if (kind == GR_TUNE_GHOST) {
+        *value = L->batch.ghost_groups;
+        return GR_OK;
+    }
+    return fail(e, GR_ERR_ARG, "unknown tuning kind for a level");
+}
+
+int gr_level_set_tuning(gr_level* L, int kind, int group, int value) {
+    if (!L) return GR_ERR_ARG;
+    gr_engine* e = L->eng;
+    if (kind == GR_TUNE_STAGE) {
+        if (group < 0 || group >= (int)L->groups.size()) return fail(e, GR_ERR_ARG, "group index out of range");
+        if (!tune_valid(kind, value)) return fail(e, GR_ERR_ARG, "unknown staging variant");
+        if (value == 1 && !L->groups[group].batch.pairs_ok)
+            return fail(e, GR_ERR_ARG, "direct LDS staging needs even row lengths and 16-byte aligned buffers");
+        L->groups[group].batch.variant = value;
+    } else if (kind == GR_TUNE_GHOST) {
+        if (!tune_valid(kind, value)) return fail(e, GR_ERR_ARG, "ghost field groups must be 1, 2, 4 or 11");
+        L->batch.ghost_groups = value;
+    } else {
+        return fail(e, GR_ERR_ARG, "unknown tuning kind for a level");
+    }
+    ++L->generation;
+    return GR_OK;
+}
+
+int gr_tune_save(const char* path) {
+    if (!path) return GR_ERR_ARG;
+    std::ofstream f(path);
+    if (!f) return GR_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    f << "# granite-ccz4 tuning table: kind class precision value\n";
+    for (const auto& kv : g_tune)
+        f << std::get<0>(kv.first) << ' ' << std::get<1>(kv.first) << ' ' << std::get<2>(kv.first) << ' ' << kv.second
+          << '\n';
+    return f ? GR_OK : GR_ERR_ARG;
+}
+
+int gr_tune_load(const char* path) {
+    if (!path) return GR_ERR_ARG;
+    std::ifstream f(path);
+    if (!f) return GR_ERR_ARG;
+    std::vector<std::pair<std::tuple<int, int, int>, int>> rows;
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t p = line.find_first_not_of(" \t\r");
+        if (p == std::string::npos || line[p] == '#') continue;
+        std::istringstream in(line);
+        int kind = -1, cls = -1, prec = -1, value = -1;
+        std::string rest;
+        if (!(in >> kind >> cls >> prec >> value) || (in >> rest)) return GR_ERR_ARG;
+        if (kind < GR_TUNE_STAGE || kind > GR_TUNE_TRANSFER || cls < 0 || cls > 3 || prec < 0 || prec > 1 ||
+            !tune_valid(kind, value))
+            return GR_ERR_ARG;
+        rows.push_back({std::make_tuple(kind, cls, prec), value});
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const auto& r : rows) g_tune[r.first] = r.second;
+    return GR_OK;
+}
+
+int gr_tune_clear(void) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.clear();
+    return GR_OK;
+}
+
+} // extern "C"
+
 // ---------------------------------------------------------------------------------------
 // Berger-Oliger hierarchy
 // ---------------------------------------------------------------------------------------
@@ -989,6 +1220,7 @@ struct gr_hierarchy {
     // pair, pair f * nparents + c at ptab_ofs[l][f * nparents + c] entries into ptab[l]
     std::vector<gr::ProlongAxis*> ptab;
     std::vector<std::vector<size_t>> ptab_ofs;
+    std::vector<int> tgroups; // per level l > 0: field groups of the transfers into / from it
 };
 
 namespace {
@@ -1013,24 +1245,31 @@ int uncovered_faces(const gr::BlockDesc& f, const std::vector<gr::BlockDesc>& pa
     return m;
 }
 
-hipError_t h_prolongate(gr_hierarchy* H, int l) {
+// prolongation into level l's states (into its first stage buffers when `to_stage`, which
+// the autotuner times without touching the state), with `groups` field groups (0: the level's)
+hipError_t h_prolongate(gr_hierarchy* H, int l, bool to_stage = false, int groups = 0) {
     gr_level* F = H->lv[l];
     gr_level* C = H->lv[l - 1];
+    const int g = groups > 0 ? groups : H->tgroups[l];
     for (size_t f = 0; f < F->blk.size(); ++f)
         for (size_t c = 0; c < C->blk.size(); ++c) {
             const gr::ProlongAxis* tab = H->ptab[l] + H->ptab_ofs[l][f * C->blk.size() + c];
-            hipError_t st = gr::launch_prolongate(C->blk[c], C->u[c], F->blk[f], F->u[f], tab, H->eng->stream);
+            hipError_t st = gr::launch_prolongate(C->blk[c], C->u[c], F->blk[f], to_stage ? F->s1[f] : F->u[f], tab,
+                                                  H->eng->stream, g);
             if (st != hipSuccess) return st;
         }
     return hipSuccess;
 }
 
-hipError_t h_restrict(gr_hierarchy* H, int l) {
+// restriction of level l into level l - 1 (into its first stage buffers when `to_stage`)
+hipError_t h_restrict(gr_hierarchy* H, int l, bool to_stage = false, int groups = 0) {
     gr_level* F = H->lv[l];
     gr_level* C = H->lv[l - 1];
+    const int g = groups > 0 ? groups : H->tgroups[l];
     for (size_t f = 0; f < F->blk.size(); ++f)
         for (size_t c = 0; c < C->blk.size(); ++c) {
-            hipError_t st = gr::launch_restrict(F->blk[f], F->u[f], C->blk[c], C->u[c], H->eng->stream);
+            hipError_t st = gr::launch_restrict(F->blk[f], F->u[f], C->blk[c], to_stage ? C->s1[c] : C->u[c],
+                                                H->eng->stream, g);
             if (st != hipSuccess) return st;
         }
     return hipSuccess;
@@ -1043,7 +1282,8 @@ hipError_t h_fill(gr_hierarchy* H, int l) {
     if (l == 0) return gr::launch_fill_ghosts_batch(L->batch, 0, GR_BC_OUTFLOW, s);
     hipError_t st = h_prolongate(H, l);
     for (size_t b = 0; st == hipSuccess && b < L->blk.size(); ++b)
-        if (H->faces[l][b]) st = gr::launch_fill_ghosts(L->blk[b], L->u[b], GR_BC_OUTFLOW, s, H->faces[l][b]);
+        if (H->faces[l][b])
+            st = gr::launch_fill_ghosts(L->blk[b], L->u[b], GR_BC_OUTFLOW, s, H->faces[l][b], L->batch.ghost_groups);
     return st;
 }
 
@@ -1102,6 +1342,11 @@ int gr_hierarchy_create(gr_engine* e, int nlevels, gr_level* const* levels, gr_h
         for (const gr::BlockDesc& f : levels[l]->blk) H->faces[l].push_back(uncovered_faces(f, levels[l - 1]->blk));
     H->ptab.assign(nlevels, nullptr);
     H->ptab_ofs.resize(nlevels);
+    H->tgroups.assign(nlevels, 4);
+    for (int l = 1; l < nlevels; ++l) {
+        int g = 0;
+        if (tune_get(GR_TUNE_TRANSFER, level_class(levels[l]), GR_PREC_FP64, &g)) H->tgroups[l] = g;
+    }
     for (int l = 1; l < nlevels; ++l) {
         size_t n = 0;
         for (const gr::BlockDesc& f : levels[l]->blk)
@@ -1285,6 +1530,62 @@ int gr_hierarchy_constraints_l2(gr_hierarchy* H, double* ham_sumsq, double* mom_
     if (ham_sumsq) *ham_sumsq = th;
     if (mom_sumsq) *mom_sumsq = tm;
     *count = n;
+    return GR_OK;
+}
+
+int gr_hierarchy_autotune(gr_hierarchy* H, const gr_ccz4_params* prm, double dt0, double* times, int cap, int* n) {
+    if (!H) return GR_ERR_ARG;
+    gr_engine* e = H->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if (cap > 0 && !times) return fail(e, GR_ERR_ARG, "null times");
+    std::vector<double> out;
+    double dt = dt0;
+    for (gr_level* L : H->lv) {
+        std::vector<double> t(2 * L->groups.size() + 4);
+        int m = 0;
+        if ((rc = gr_level_autotune(L, prm, dt, t.data(), (int)t.size(), &m))) return rc;
+        out.insert(out.end(), t.begin(), t.begin() + m);
+        dt = dt / 2.0;
+    }
+    for (int l = 1; l < (int)H->lv.size(); ++l) {
+        int best = H->tgroups[l];
+        double tbest = INFINITY;
+        for (int g : kGroupChoices) {
+            double ms = 0.0;
+            // into the stage buffers: the states are untouched
+            GR_HIP(e, time_launches(e, 3, [&] {
+                hipError_t st = h_prolongate(H, l, true, g);
+                return st == hipSuccess ? h_restrict(H, l, true, g) : st;
+            }, &ms));
+            out.push_back(ms);
+            if (ms < tbest) {
+                tbest = ms;
+                best = g;
+            }
+        }
+        H->tgroups[l] = best;
+        tune_put(GR_TUNE_TRANSFER, level_class(H->lv[l]), GR_PREC_FP64, best);
+    }
+    H->have_graph = false; // the captured coarse step holds the launch forms
+    for (int i = 0; i < cap && i < (int)out.size(); ++i) times[i] = out[i];
+    if (n) *n = (int)std::min<size_t>(out.size(), (size_t)std::max(cap, 0));
+    return GR_OK;
+}
+
+int gr_hierarchy_get_tuning(gr_hierarchy* H, int level, int* groups) {
+    if (!H) return GR_ERR_ARG;
+    if (!groups || level < 1 || level >= (int)H->lv.size()) return fail(H->eng, GR_ERR_ARG, "level must be 1 .. n-1");
+    *groups = H->tgroups[level];
+    return GR_OK;
+}
+
+int gr_hierarchy_set_tuning(gr_hierarchy* H, int level, int groups) {
+    if (!H) return GR_ERR_ARG;
+    if (level < 1 || level >= (int)H->lv.size()) return fail(H->eng, GR_ERR_ARG, "level must be 1 .. n-1");
+    if (!tune_valid(GR_TUNE_TRANSFER, groups)) return fail(H->eng, GR_ERR_ARG, "transfer field groups must be 1, 2, 4 or 11");
+    H->tgroups[level] = groups;
+    H->have_graph = false;
     return GR_OK;
 }
 

@@ -50,21 +50,13 @@
 // (55 KB for two buffers of an 8x4x4 part: two workgroups per CU); 1 = unpadded rows,
 // planes padded, lanes interleaved over y and z (38 KB: room for four workgroups per CU,
 // i.e. two waves per SIMD when the registers allow it)
-#ifndef GR_FP64_LAYOUT
-#define GR_FP64_LAYOUT 0
-#endif
-// 1 = FP64 halo staged by direct global -> LDS loads (global_load_lds_dwordx4, layout 1
-// only): no registers hold the prefetched field
-#ifndef GR_FP64_DMA
-#define GR_FP64_DMA 0
-#endif
 #ifndef GR_MIXED_PREFETCH
 #define GR_MIXED_PREFETCH 1
 #endif
 #ifndef GR_LDS_PAD32
 #define GR_LDS_PAD32 1
 #endif
-// field groups (second grid dimension) of the ghost fill
+// default field groups (second grid dimension) of the ghost fill (1, 2, 4 or 11)
 #ifndef GR_GHOST_GROUPS
 #define GR_GHOST_GROUPS 4
 #endif
@@ -296,7 +288,7 @@ struct TileGeom {
 // with that field's prefetch.
 template <int TX, int TY, int TZ, int MODE, int PF>
 struct TileIO {
-    using T = TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>;
+    using T = TileGeom<TX, TY, TZ, true, 8, 0>;
     static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
     static constexpr int LBUF = T::NLDS + 1; // one LDS buffer incl. a discard slot
 
@@ -749,9 +741,9 @@ __device__ __forceinline__ void load_sources(const Sources& S, long cidx, CellSo
 
 // part `part` of a tile: TX x TY x TZ cells at x = (part % GR_FP64_SPLIT) * TX,
 // z = (part / GR_FP64_SPLIT) * TZ inside the tile
-template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
+template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT, int SV>
 __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R, int part, double* lds) {
-    using T = TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>;
+    using T = TileGeom<TX, TY, TZ, true, 8, SV>;
     const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
     int lx = 0, ly = 0, lz = 0;
@@ -763,12 +755,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
     const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
     const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
 
-#if GR_FP64_DMA
-    static_assert(GR_FP64_LAYOUT == 1, "direct LDS staging needs LDS layout 1");
-    TileIODma<TX, TY, TZ, MODE> io;
-#else
-    TileIO<TX, TY, TZ, MODE, GR_FP64_PREFETCH> io;
-#endif
+    typename std::conditional<SV == 1, TileIODma<TX, TY, TZ, MODE>, TileIO<TX, TY, TZ, MODE, GR_FP64_PREFETCH>>::type io;
     io.lds = lds;
     io.in = R.in;
     io.un = R.un;
@@ -790,11 +777,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
 // parts of a tile in the FP64 kernels
 constexpr int FP64_PARTS = GR_FP64_SPLIT * GR_FP64_ZSPLIT;
 
-// LDS doubles of one FP64 part's two field buffers
-template <int TX, int TY, int TZ>
+// LDS doubles of one FP64 part's two field buffers under staging variant SV
+template <int TX, int TY, int TZ, int SV>
 constexpr int fp64_lds_doubles() {
-    return GR_FP64_DMA ? 2 * TileIODma<TX, TY, TZ, MODE_RHS>::LBUF
-                       : 2 * (TileGeom<TX, TY, TZ, true, 8, GR_FP64_LAYOUT>::NLDS + 1);
+    return SV == 1 ? 2 * TileIODma<TX, TY, TZ, MODE_RHS>::LBUF : 2 * (TileGeom<TX, TY, TZ, true, 8, 0>::NLDS + 1);
 }
 
 // whether part `part` of tile R holds no cell of the block (uniform over the workgroup)
@@ -805,24 +791,24 @@ __device__ __forceinline__ bool part_outside(const TileRef& R, int part) {
 }
 
 // one part of one tile per workgroup (FP64_PARTS consecutive workgroups per tile)
-template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
+template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT, int SV>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
-    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ>()];
+    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ, SV>()];
     const int slot = tile_of_block(blockIdx.x, A.ntiles * FP64_PARTS, A.xcd_remap);
     const int tile = slot / FP64_PARTS, part = slot % FP64_PARTS;
     if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
     const TileRef R = resolve_tile(A, tile);
     if (part_outside<TX, TZ>(R, part)) return;
-    stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, R, part, lds);
+    stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT, SV>(A, R, part, lds);
 }
 
 // the flagged tiles only, from a small grid: each workgroup gathers the tiles with
 // flags == want among tiles blockIdx.x + k * gridDim.x and runs them one after the other
 // (every thread takes the same trip counts, so the barriers are uniform)
-template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT>
+template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT, int SV>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_scan(StageArgs A) {
     constexpr int NT = TX * TY * TZ;
-    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ>()];
+    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ, SV>()];
     __shared__ int list[NT];
     __shared__ int nlist;
     const int per = (int)gridDim.x * NT;
@@ -838,7 +824,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_sc
             const TileRef R = resolve_tile(A, list[item]);
             for (int part = 0; part < FP64_PARTS; ++part) {
                 if (part_outside<TX, TZ>(R, part)) continue; // uniform over the workgroup
-                stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT>(A, R, part, lds);
+                stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT, SV>(A, R, part, lds);
                 __syncthreads(); // the next part reuses the LDS buffers
             }
         }
@@ -850,12 +836,11 @@ static_assert(GR_TILE_X % GR_FP64_SPLIT == 0 && GR_TILE_Z % GR_FP64_ZSPLIT == 0,
 constexpr int TILE_X = GR_TILE_X, TILE_Y = GR_TILE_Y, TILE_Z = GR_TILE_Z;
 constexpr int FP64_TX = GR_TILE_X / GR_FP64_SPLIT, FP64_TY = GR_TILE_Y, FP64_TZ = GR_TILE_Z / GR_FP64_ZSPLIT;
 static_assert(!GR_LDS_PAD || FP64_TX != 8 || TILE_X != 16 || TILE_Y != 4 ||
-                  ((GR_FP64_LAYOUT ? (TileGeom<FP64_TX, FP64_TY, FP64_TZ, true, 8, 1>::BX == 16 &&
-                                      TileGeom<FP64_TX, FP64_TY, FP64_TZ, true, 8, 1>::PLANE == 200)
-                                   : TileGeom<FP64_TX, FP64_TY, FP64_TZ>::BX == 24) &&
-                   TileGeom<TILE_X, TILE_Y, TILE_Z, true, 4>::BX == 48),
-              "LDS layouts of the default tiles: FP64 rows of 16 and planes of 200 (layout 1) or rows of 24 "
-              "(layout 0); FP32 rows of 48");
+                  (TileGeom<FP64_TX, FP64_TY, FP64_TZ, true, 8, 1>::BX == 16 &&
+                   TileGeom<FP64_TX, FP64_TY, FP64_TZ, true, 8, 1>::PLANE == 200 &&
+                   TileGeom<FP64_TX, FP64_TY, FP64_TZ>::BX == 24 && TileGeom<TILE_X, TILE_Y, TILE_Z, true, 4>::BX == 48),
+              "LDS layouts of the default tiles: FP64 rows of 24 (layout 0), rows of 16 in planes of 200 (layout "
+              "1); FP32 rows of 48");
 // workgroups of the FP64 pass of the mixed form: 256 threads per CU
 constexpr int FP64_SCAN_GRID = 256 * 256 / (FP64_TX * FP64_TY * FP64_TZ);
 
@@ -916,16 +901,22 @@ __global__ __launch_bounds__(TX* TY* TZ) void classify_tiles_kernel(StageArgs A,
     if (tid == 0) flags[tile] = (unsigned char)bad;
 }
 
-template <int MODE, bool HAS_SRC>
-hipError_t launch_stage_t(const StageArgs& A, int nblk, hipStream_t s) {
-    if (nblk < 1) return hipSuccess;
+template <int MODE, bool HAS_SRC, int SV>
+hipError_t launch_stage_sv(const StageArgs& A, int nblk, hipStream_t s) {
     if (A.scan)
-        stage_kernel_scan<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0>
+        stage_kernel_scan<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0, SV>
             <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
     else
-        stage_kernel<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0>
+        stage_kernel<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0, SV>
             <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
     return hipGetLastError();
+}
+
+template <int MODE, bool HAS_SRC>
+hipError_t launch_stage_t(const StageArgs& A, int nblk, int variant, hipStream_t s) {
+    if (nblk < 1) return hipSuccess;
+    if (variant == 1) return launch_stage_sv<MODE, HAS_SRC, 1>(A, nblk, s);
+    return launch_stage_sv<MODE, HAS_SRC, 0>(A, nblk, s);
 }
 
 // ------------------------------------------------------------------------------------
@@ -960,8 +951,9 @@ __device__ __forceinline__ long ghost_count(const BlockDesc& b) {
 // (bit 2d: low side of axis d, bit 2d+1: high side); the others keep their values, as
 // fillGhostZones leaves the faces covered by a parent level to prolongation
 // (src/amr/amr.cpp:412-480).
-// Fields g, g + GR_GHOST_GROUPS, ... of the cell: the field groups run as separate
-// workgroups (second grid dimension), so a small ghost region still fills the machine.
+// Fields g, g + G, ... of the cell: the G field groups run as separate workgroups (second
+// grid dimension), so a small ghost region still fills the machine.
+template <int G>
 __device__ void fill_ghost_cell(const BlockDesc& b, gdouble* u, long t_, int mode, int faces, int g) {
 #pragma clang fp contract(off)
     // ghost cells number fewer than 2^29 (the block limit): 32-bit index arithmetic
@@ -1011,11 +1003,11 @@ __device__ void fill_ghost_cell(const BlockDesc& b, gdouble* u, long t_, int mod
     const long dst = I[0] + (long)nt0 * (I[1] + (long)nt1 * I[2]);
     const long src = A[0] + (long)nt0 * (A[1] + (long)nt1 * A[2]);
     // the anchor is never a cell this launch writes, so all loads go out before any store
-    constexpr int NF = (NV + GR_GHOST_GROUPS - 1) / GR_GHOST_GROUPS;
+    constexpr int NF = (NV + G - 1) / G;
     double val[NF];
 #pragma unroll
     for (int m = 0; m < NF; ++m) {
-        const int v = g + GR_GHOST_GROUPS * m;
+        const int v = g + G * m;
         if (v < NV) val[m] = u[v * b.stride + src];
     }
     if (mode == 0) {
@@ -1030,22 +1022,23 @@ __device__ void fill_ghost_cell(const BlockDesc& b, gdouble* u, long t_, int mod
             const double rgh = radius(Q[0], Q[1], Q[2]);
 #pragma unroll
             for (int m = 0; m < NF; ++m) {
-                const double fi = f_inf(g + GR_GHOST_GROUPS * m);
+                const double fi = f_inf(g + G * m);
                 val[m] = fi + (val[m] - fi) * rin / rgh;
             }
         }
     }
 #pragma unroll
     for (int m = 0; m < NF; ++m) {
-        const int v = g + GR_GHOST_GROUPS * m;
+        const int v = g + G * m;
         if (v < NV) u[v * b.stride + dst] = val[m];
     }
 }
 
+template <int G>
 __global__ void fill_ghosts_kernel(BlockDesc b, double* u, int mode, int faces) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ghost_count(b)) return;
-    fill_ghost_cell(b, (gdouble*)u, t, mode, faces, blockIdx.y);
+    fill_ghost_cell<G>(b, (gdouble*)u, t, mode, faces, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1230,12 +1223,13 @@ __global__ void restrict_kernel(BlockDesc f, const double* __restrict__ fu, Bloc
 
 // every ghost cell of buffer `sel` of every block of a table; ofs[i] = first ghost cell of
 // block i (nblocks + 1 entries)
+template <int G>
 __global__ void fill_ghosts_batch_kernel(const BatchBlock* __restrict__ table, const int* __restrict__ wg_block,
                                          int sel, int mode) {
     const BatchBlock& B = table[wg_block[blockIdx.x]]; // one block per workgroup
     const long t = (long)(blockIdx.x - B.ghost_wg0) * GHOST_WG + threadIdx.x;
     if (t >= ghost_count(B.b)) return;
-    fill_ghost_cell(B.b, (gdouble*)B.buf[sel], t, mode, 0x3f, blockIdx.y);
+    fill_ghost_cell<G>(B.b, (gdouble*)B.buf[sel], t, mode, 0x3f, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1773,6 +1767,8 @@ bool pairs_aligned(const BlockDesc& b, const void* p) {
 Batch single_batch(const BlockDesc& b, const double* in, const double* un, double* out, const Sources& src) {
     Batch B;
     B.pairs_ok = pairs_aligned(b, in) && pairs_aligned(b, un) && pairs_aligned(b, out);
+    B.variant = 0;
+    B.ghost_groups = GR_GHOST_GROUPS;
     B.table = nullptr;
     B.tile_block = nullptr;
     B.ghost_wg_block = nullptr;
@@ -1796,8 +1792,8 @@ Batch single_batch(const BlockDesc& b, const double* in, const double* un, doubl
 hipError_t launch_stage_fp64(const Batch& B, const int sel[3], bool has_src, const CellParams& P,
                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags) {
-    // the direct LDS staging reads 16-byte pairs
-    if (GR_FP64_DMA && !B.pairs_ok) return hipErrorInvalidValue;
+    // the direct LDS staging reads 16-byte pairs: other batches run the register staging
+    const int variant = (B.variant == 1 && B.pairs_ok) ? 1 : 0;
     StageArgs A = stage_args(B, sel, P, ko_sigma, cf, flags, 1);
     // with flags (the FP64 tiles of the mixed form: few, near the puncture) a small grid
     // gathers them instead of launching one workgroup per tile of the level
@@ -1807,13 +1803,13 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], bool has_src, con
         nblk = A.ntiles < FP64_SCAN_GRID ? A.ntiles : FP64_SCAN_GRID;
     }
     if (has_src) {
-        if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, true>(A, nblk, s);
-        if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, true>(A, nblk, s);
-        return launch_stage_t<MODE_FINAL, true>(A, nblk, s);
+        if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, true>(A, nblk, variant, s);
+        if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, true>(A, nblk, variant, s);
+        return launch_stage_t<MODE_FINAL, true>(A, nblk, variant, s);
     }
-    if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, false>(A, nblk, s);
-    if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, false>(A, nblk, s);
-    return launch_stage_t<MODE_FINAL, false>(A, nblk, s);
+    if (mode == MODE_RHS) return launch_stage_t<MODE_RHS, false>(A, nblk, variant, s);
+    if (mode == MODE_STAGE) return launch_stage_t<MODE_STAGE, false>(A, nblk, variant, s);
+    return launch_stage_t<MODE_FINAL, false>(A, nblk, variant, s);
 }
 
 template <int MODE, bool HAS_SRC>
@@ -1856,19 +1852,40 @@ hipError_t launch_stage_mixed(const Batch& B, const int sel[3], bool has_src, co
     return launch_stage_fp64(B, sel, has_src, P, ko_sigma, mode, cf, s, flags);
 }
 
+bool ghost_groups_valid(int g) { return g == 1 || g == 2 || g == 4 || g == 11; }
+
+template <int G>
+static void fill_batch_g(const Batch& B, int sel, int mode, hipStream_t s) {
+    fill_ghosts_batch_kernel<G><<<dim3(B.ghost_wgs, G), GHOST_WG, 0, s>>>(B.table, B.ghost_wg_block, sel, mode);
+}
+
 hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream_t s) {
-    if (!B.table) return launch_fill_ghosts(B.one.b, B.one.buf[sel], mode, s, 0x3f);
-    fill_ghosts_batch_kernel<<<dim3(B.ghost_wgs, GR_GHOST_GROUPS), GHOST_WG, 0, s>>>(B.table, B.ghost_wg_block, sel,
-                                                                                  mode);
+    if (!B.table) return launch_fill_ghosts(B.one.b, B.one.buf[sel], mode, s, 0x3f, B.ghost_groups);
+    switch (B.ghost_groups) {
+    case 1: fill_batch_g<1>(B, sel, mode, s); break;
+    case 2: fill_batch_g<2>(B, sel, mode, s); break;
+    case 11: fill_batch_g<11>(B, sel, mode, s); break;
+    default: fill_batch_g<4>(B, sel, mode, s); break;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s, int faces) {
+template <int G>
+static void fill_one_g(const BlockDesc& b, double* u, int mode, hipStream_t s, int faces, long n) {
+    fill_ghosts_kernel<G><<<dim3(grid_for(n, GHOST_WG), G), GHOST_WG, 0, s>>>(b, u, mode, faces);
+}
+
+hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s, int faces, int groups) {
     const long nA = 2L * b.ng * b.nt[1] * b.nt[0];
     const long nB = (long)b.n[2] * 2 * b.ng * b.nt[0];
     const long nC = (long)b.n[2] * b.n[1] * 2 * b.ng;
     const long n = nA + nB + nC;
-    fill_ghosts_kernel<<<dim3(grid_for(n, GHOST_WG), GR_GHOST_GROUPS), GHOST_WG, 0, s>>>(b, u, mode, faces);
+    switch (groups) {
+    case 1: fill_one_g<1>(b, u, mode, s, faces, n); break;
+    case 2: fill_one_g<2>(b, u, mode, s, faces, n); break;
+    case 11: fill_one_g<11>(b, u, mode, s, faces, n); break;
+    default: fill_one_g<4>(b, u, mode, s, faces, n); break;
+    }
     return hipGetLastError();
 }
 
@@ -1878,15 +1895,18 @@ hipError_t launch_prolong_axes(const BlockDesc& c, const BlockDesc& f, ProlongAx
 }
 
 hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu,
-                             const ProlongAxis* tab, hipStream_t s) {
+                             const ProlongAxis* tab, hipStream_t s, int groups) {
     const int ntx = (f.nt[0] + PT_X - 1) / PT_X, nty = (f.nt[1] + PT_Y - 1) / PT_Y, ntz = (f.nt[2] + PT_Z - 1) / PT_Z;
-    prolong_kernel<<<dim3(ntx * nty * ntz, GR_TRANSFER_GROUPS), PT_N, 0, s>>>(c, cu, f, fu, tab, ntx, nty);
+    const int g = groups > 0 && groups <= NV ? groups : GR_TRANSFER_GROUPS;
+    prolong_kernel<<<dim3(ntx * nty * ntz, g), PT_N, 0, s>>>(c, cu, f, fu, tab, ntx, nty);
     return hipGetLastError();
 }
 
-hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc& c, double* cu, hipStream_t s) {
+hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc& c, double* cu, hipStream_t s,
+                           int groups) {
     const long n = (long)c.n[0] * c.n[1] * c.n[2];
-    restrict_kernel<<<dim3(grid_for(n, 256), GR_TRANSFER_GROUPS), 256, 0, s>>>(f, fu, c, cu);
+    const int g = groups > 0 && groups <= NV ? groups : GR_TRANSFER_GROUPS;
+    restrict_kernel<<<dim3(grid_for(n, 256), g), 256, 0, s>>>(f, fu, c, cu);
     return hipGetLastError();
 }
 

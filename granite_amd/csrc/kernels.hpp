@@ -51,6 +51,11 @@ constexpr int GHOST_WG = 256;
 // workgroup w; each block's ghost cells start on a workgroup of their own) or the single
 // block `one`. sel_* pick the buffer of
 // each block that is the stage input, u^n and the output (0, 1, 2 = buf[0..2]).
+// FP64 staging variants of the stage kernel (chosen per launch, by the autotuner): 0 = halo
+// prefetched into registers and written to a padded LDS box (one wave per SIMD); 1 = halo
+// loaded straight into an unpadded LDS box (two waves per SIMD), for even row lengths
+constexpr int FP64_STAGE_VARIANTS = 2;
+
 struct Batch {
     const BatchBlock* table;
     const int* tile_block;
@@ -61,6 +66,8 @@ struct Batch {
     int ghost_wgs; // workgroups of a ghost launch
     double dx[3]; // grid spacing shared by all blocks of the launch
     int pairs_ok; // every block: even row length (nt0) and 16-byte aligned buffers
+    int variant;  // FP64 staging variant (0 registers, 1 direct LDS; 1 needs pairs_ok)
+    int ghost_groups; // field groups of the ghost fill launch (1, 2, 4 or 11)
 };
 // whether block b's rows can be read as 16-byte pairs from buffer p
 bool pairs_aligned(const BlockDesc& b, const void* p);
@@ -111,7 +118,10 @@ hipError_t launch_fill_ghosts_batch(const Batch& B, int sel, int mode, hipStream
 
 // Ghost fill of one block: mode 0 Sommerfeld (chained X,Y,Z), 1 periodic, 2 outflow copy
 // of the faces set in `faces` (bit 2d low side, 2d+1 high side of axis d).
-hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s, int faces = 0x3f);
+hipError_t launch_fill_ghosts(const BlockDesc& b, double* u, int mode, hipStream_t s, int faces = 0x3f,
+                              int groups = 4);
+// field-group counts the ghost fill is compiled for
+bool ghost_groups_valid(int g);
 
 // one axis of the trilinear coarse -> fine stencil at one fine index: the two coarse
 // neighbours and the weight of c1
@@ -126,8 +136,9 @@ hipError_t launch_prolong_axes(const BlockDesc& c, const BlockDesc& f, ProlongAx
 // trilinear coarse -> fine into every fine cell (ghosts included) from that table, and 2x2x2
 // average of the fine block into the covered coarse interior cells
 hipError_t launch_prolongate(const BlockDesc& c, const double* cu, const BlockDesc& f, double* fu,
-                             const ProlongAxis* tab, hipStream_t s);
-hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc& c, double* cu, hipStream_t s);
+                             const ProlongAxis* tab, hipStream_t s, int groups = 0);
+hipError_t launch_restrict(const BlockDesc& f, const double* fu, const BlockDesc& c, double* cu, hipStream_t s,
+                           int groups = 0);
 
 // rhs += Kreiss-Oliger(u) on the interior (reference accumulation order)
 hipError_t launch_dissipation(const BlockDesc& b, const double* u, double* rhs, double ko_sigma, hipStream_t s);

@@ -309,6 +309,44 @@ int gr_level_profile(gr_level* lvl, int enable);
  * kernel (with the algebraic constraints), 2 = ghost fills. */
 int gr_level_profile_read(gr_level* lvl, int kind, double* total_ms, int* launches);
 
+/* ---- autotuning (no reference counterpart: the reference's only schedule is a fixed OpenMP
+ * collapse(3) schedule(static), src/spacetime/ccz4.cpp:266-268) ---------------------------- */
+
+/* Tuned choices. GR_TUNE_STAGE: the FP64 stage kernel's halo staging, 0 = prefetch through
+ * registers into a padded LDS box (one wave per SIMD), 1 = direct global-to-LDS loads into an
+ * unpadded box (two waves per SIMD; blocks with an even row length only, others run 0).
+ * GR_TUNE_GHOST / GR_TUNE_TRANSFER: field groups (1, 2, 4 or 11) of the ghost fill and of the
+ * prolongation and restriction launches. */
+enum gr_tune_kind { GR_TUNE_STAGE = 0, GR_TUNE_GHOST = 1, GR_TUNE_TRANSFER = 2 };
+
+/* Times every variant of the stage kernel (per group of equal spacing: stage 1 into the
+ * first stage buffer, the state untouched) and of the ghost fill (into that buffer's ghosts)
+ * on this level's own blocks, keeps the fastest in the level and in the process-wide tuning
+ * table under the level's block-size class (mean interior cells per block: < 24^3, < 48^3,
+ * < 96^3, larger). times (optional, cap entries) receives the measured ms per launch in the
+ * order: for each group, each stage variant (-1 where not applicable); then each ghost group
+ * count (1, 2, 4, 11); *n = entries written. Captured graphs of the level are re-recorded. */
+int gr_level_autotune(gr_level* lvl, const gr_ccz4_params* prm, double dt, double* times, int cap, int* n);
+/* gr_level_autotune of every level (dt0 / 2^l), then the transfer field groups of every
+ * level l > 0 (prolongation into the level's first stage buffer, restriction into the
+ * parent's, the states untouched). times: the levels' entries, then per level l > 0 the
+ * four group counts. */
+int gr_hierarchy_autotune(gr_hierarchy* h, const gr_ccz4_params* prm, double dt0, double* times, int cap, int* n);
+/* The choice in force for group `group` of the level (GR_TUNE_STAGE; group < number of
+ * groups of equal spacing) or for the level (GR_TUNE_GHOST), and forcing one (tests and
+ * experiments; an unknown value fails with GR_ERR_ARG). */
+int gr_level_get_tuning(gr_level* lvl, int kind, int group, int* value);
+int gr_level_set_tuning(gr_level* lvl, int kind, int group, int value);
+/* Transfer field groups of level l > 0 of a hierarchy (get / force). */
+int gr_hierarchy_get_tuning(gr_hierarchy* h, int level, int* groups);
+int gr_hierarchy_set_tuning(gr_hierarchy* h, int level, int groups);
+/* The process-wide tuning table: write to / read from a text file (one line per entry:
+ * kind class precision value), or empty it. Levels and hierarchies created afterwards use
+ * the entries of their class; absent entries keep the defaults (staging 0, 4 groups). */
+int gr_tune_save(const char* path);
+int gr_tune_load(const char* path);
+int gr_tune_clear(void);
+
 /* ---- introspection ------------------------------------------------------------------- */
 
 /* Stage-kernel tile shape in cells (x, y, z) used for the given precision: FP64, the part of
