@@ -106,7 +106,8 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "copies": 5}), flush=True)
 
 
-STAGING = {0: "registers -> padded LDS", 1: "direct global -> LDS"}
+STAGING = {0: "registers -> padded LDS, 8x4x4 parts", 1: "direct global -> LDS, 8x4x4 parts",
+           2: "direct global -> LDS, 8x4x8 parts"}
 
 
 def level_tuning(ga, lvl, times=None):
@@ -133,21 +134,29 @@ def run_batched(ga, eng, p, args):
     steps = max(3, args.steps // 5)
     out = {"blocks": sum(len(b) for _, b in layout), "levels": len(layout), "cells": total}
 
-    def timed(levels, precision, tune):
+    def timed(levels, precision, tune, together=False):
         tuning = []
         for lv, dx in levels:
             lv.init_brill_lindquist(masses, pos)
             if tune and not args.no_tune:
                 tuning.append(level_tuning(ga, lv, lv.autotune(p, 0.25 * dx)))
+        def step():
+            if together:
+                # the level set as one schedule: each level's ghost fills overlap the
+                # following levels' stage kernels (gr_levels_rk3_step)
+                ga.levels_rk3_step([lv for lv, _ in levels], [0.25 * dx for _, dx in levels], p, precision=precision,
+                                   pre_fill=ga.BC_OUTFLOW, use_graph=True)
+            else:
+                for lv, dx in levels:
+                    lv.rk3_step(p, 0.25 * dx, precision=precision, pre_fill=ga.BC_OUTFLOW, use_graph=True)
+
         for _ in range(2):
-            for lv, dx in levels:
-                lv.rk3_step(p, 0.25 * dx, precision=precision, pre_fill=ga.BC_OUTFLOW, use_graph=True)
+            step()
         for lv, _ in levels:
             lv.sync()
         t0 = time.perf_counter()
         for _ in range(steps):
-            for lv, dx in levels:
-                lv.rk3_step(p, 0.25 * dx, precision=precision, pre_fill=ga.BC_OUTFLOW, use_graph=True)
+            step()
         for lv, _ in levels:
             lv.sync()
         ms = (time.perf_counter() - t0) * 1e3 / steps
@@ -167,13 +176,17 @@ def run_batched(ga, eng, p, args):
                 levels += [(ga.Level([g], eng), dx) for g in grids]
         return levels
 
-    for name, batched, prec in (("fp64_batched", True, ga.PREC_FP64), ("mixed_batched", True, ga.PREC_MIXED),
-                                ("fp64_per_block", False, ga.PREC_FP64)):
+    for name, batched, prec, together in (("fp64_batched", True, ga.PREC_FP64, True),
+                                          ("mixed_batched", True, ga.PREC_MIXED, True),
+                                          ("fp64_batched_levels_one_by_one", True, ga.PREC_FP64, False),
+                                          ("fp64_per_block", False, ga.PREC_FP64, False)):
         levels = make(batched)
-        out[name] = timed(levels, prec, batched)
+        out[name] = timed(levels, prec, batched, together)
         for lv, _ in levels:
             lv.close()
     out["batched_speedup_vs_per_block"] = out["fp64_per_block"]["ms_per_step"] / out["fp64_batched"]["ms_per_step"]
+    out["level_set_speedup_vs_one_by_one"] = (out["fp64_batched_levels_one_by_one"]["ms_per_step"] /
+                                              out["fp64_batched"]["ms_per_step"])
     out["mixed_batched_vs_fp64_per_block"] = out["fp64_per_block"]["ms_per_step"] / out["mixed_batched"]["ms_per_step"]
     out["steps"] = steps
     return out

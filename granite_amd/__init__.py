@@ -28,6 +28,7 @@ from .ccz4 import (  # noqa: F401
     SpacetimeVar,
     library,
     library_path,
+    levels_rk3_step,
     tune_clear,
     tune_load,
     tune_save,

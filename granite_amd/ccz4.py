@@ -22,6 +22,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 # tuned choices (enum gr_tune_kind)
 TUNE_STAGE, TUNE_GHOST, TUNE_TRANSFER = 0, 1, 2
 GROUP_CHOICES = (1, 2, 4, 11)
+STAGE_VARIANTS = 3  # FP64 staging: 0 registers, 1 direct LDS (8x4x4 parts), 2 direct LDS (8x4x8 parts)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -152,6 +153,8 @@ def library():
         "gr_tune_save": (C.c_int, [C.c_char_p]),
         "gr_tune_load": (C.c_int, [C.c_char_p]),
         "gr_tune_clear": (C.c_int, []),
+        "gr_levels_rk3_step": (C.c_int, [C.c_int, C.POINTER(vp), _dp, C.POINTER(_Params), C.c_int, C.c_int, C.c_int,
+                                         C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -167,6 +170,19 @@ def _ptr(a):
     if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
         raise TypeError("expected a C-contiguous float64 numpy array")
     return a.ctypes.data_as(_dp)
+
+
+def levels_rk3_step(levels, dts, params: "CCZ4Params", precision=0, bc_mode=0, pre_fill=-1, use_graph=False):
+    """gr_levels_rk3_step: one RK3 step of independent levels (level i with dts[i]), each
+    level's ghost fills overlapping the next levels' stage kernels."""
+    levels = list(levels)
+    if not levels:
+        return
+    eng = levels[0]._e
+    arr = (C.c_void_p * len(levels))(*[lv._h for lv in levels])
+    d = (C.c_double * len(levels))(*[float(x) for x in dts])
+    eng.check(eng.lib.gr_levels_rk3_step(len(levels), arr, d, C.byref(params.c()), int(precision), int(bc_mode),
+                                         int(pre_fill), 1 if use_graph else 0))
 
 
 def tune_save(path):
@@ -484,7 +500,8 @@ class Level:
         self._e.check(self._e.lib.gr_level_autotune(self._h, C.byref(params.c()), float(dt), t, cap, C.byref(n)))
         vals = [t[i] for i in range(n.value)]
         ng = len(vals) - len(GROUP_CHOICES)
-        stage = [[None if v < 0 else v for v in vals[i:i + 2]] for i in range(0, ng, 2)]
+        nv = STAGE_VARIANTS
+        stage = [[None if v < 0 else v for v in vals[i:i + nv]] for i in range(0, ng, nv)]
         return {"stage": stage, "ghost": dict(zip(GROUP_CHOICES, vals[ng:]))}
 
     def tuning(self, kind=TUNE_STAGE, group=0):

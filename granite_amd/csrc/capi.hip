@@ -42,6 +42,13 @@ struct gr_engine {
     double* scratch = nullptr; // device address of `scratch` on this engine's device
     gr::AuxStream aux{nullptr, nullptr, nullptr}; // runs the FP64 tiles beside the FP32 ones
     std::string err;
+    // multi-level step: a second stream for the ghost fills, events ordering each level's
+    // stages and fills, and the captured graph of the last level set
+    hipStream_t fill_stream = nullptr;
+    std::vector<hipEvent_t> ls_events;
+    hipGraph_t ls_graph = nullptr;
+    hipGraphExec_t ls_exec = nullptr;
+    std::vector<double> ls_key;
     // compatibility-mode staging buffers (grown on demand, doubles)
     double* d_u = nullptr;
     double* d_r = nullptr;
@@ -246,6 +253,13 @@ int gr_engine_destroy(gr_engine* e) {
     if (e->aux.fork) (void)hipEventDestroy(e->aux.fork);
     if (e->aux.join) (void)hipEventDestroy(e->aux.join);
     if (e->aux.stream) (void)hipStreamDestroy(e->aux.stream);
+    if (e->ls_exec) (void)hipGraphExecDestroy(e->ls_exec);
+    if (e->ls_graph) (void)hipGraphDestroy(e->ls_graph);
+    for (hipEvent_t ev : e->ls_events) (void)hipEventDestroy(ev);
+    if (e->fill_stream) {
+        (void)hipStreamSynchronize(e->fill_stream);
+        (void)hipStreamDestroy(e->fill_stream);
+    }
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return GR_OK;
@@ -536,6 +550,13 @@ int gr_level_destroy(gr_level* L) {
     if (e) {
         (void)hipSetDevice(e->device);
         (void)hipStreamSynchronize(e->stream);
+        if (e->fill_stream) (void)hipStreamSynchronize(e->fill_stream);
+        // a captured level-set step may hold this level's buffers
+        if (e->ls_exec) (void)hipGraphExecDestroy(e->ls_exec);
+        if (e->ls_graph) (void)hipGraphDestroy(e->ls_graph);
+        e->ls_exec = nullptr;
+        e->ls_graph = nullptr;
+        e->ls_key.clear();
     }
     if (L->exec) (void)hipGraphExecDestroy(L->exec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
@@ -716,6 +737,42 @@ static hipError_t enqueue_rk3(gr_level* L, const gr_ccz4_params* prm, double dt,
     return st;
 }
 
+// One RK3 step of n independent levels as one schedule over two streams: the stage launches
+// go to the engine stream in level order, stage by stage; each level's ghost fill goes to
+// the fill stream once its stage has run (event), and the level's next stage waits for that
+// fill (event). A level's fill thus overlaps the following levels' stage kernels, while each
+// level sees exactly the launches of its own step in the same order.
+static hipError_t enqueue_levels_rk3(gr_engine* e, int n, gr_level* const* lv, const double* dts,
+                                     const gr_ccz4_params* prm, int bc, int pre, int precision) {
+    const size_t need = 2 * 3 * (size_t)n + 1;
+    while (e->ls_events.size() < need) {
+        hipEvent_t ev = nullptr;
+        hipError_t st = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (st != hipSuccess) return st;
+        e->ls_events.push_back(ev);
+    }
+    hipEvent_t* staged = e->ls_events.data();          // [level][stage]
+    hipEvent_t* filled = e->ls_events.data() + 3 * n;  // [level][stage]
+    hipEvent_t fork = e->ls_events[6 * (size_t)n];
+    hipStream_t A = e->stream, B = e->fill_stream;
+    hipError_t st = hipEventRecord(fork, A);
+    if (st == hipSuccess) st = hipStreamWaitEvent(B, fork, 0); // B joins A's order (and a capture)
+    for (int k = 1; k <= 3 && st == hipSuccess; ++k)
+        for (int i = 0; i < n && st == hipSuccess; ++i) {
+            gr_level* L = lv[i];
+            if (k > 1) st = hipStreamWaitEvent(A, filled[3 * i + k - 2], 0);
+            if (st == hipSuccess && k == 1 && pre >= 0) st = gr::launch_fill_ghosts_batch(L->batch, 0, pre, A);
+            const int mode = k == 3 ? gr::MODE_FINAL : gr::MODE_STAGE;
+            if (st == hipSuccess) st = stage(L, kStageSel[k], prm, mode, stage_coef(k, dts[i]), precision);
+            if (st == hipSuccess) st = hipEventRecord(staged[3 * i + k - 1], A);
+            if (st == hipSuccess) st = hipStreamWaitEvent(B, staged[3 * i + k - 1], 0);
+            if (st == hipSuccess) st = gr::launch_fill_ghosts_batch(L->batch, kStageSel[k][2], bc, B);
+            if (st == hipSuccess) st = hipEventRecord(filled[3 * i + k - 1], B);
+        }
+    for (int i = 0; i < n && st == hipSuccess; ++i) st = hipStreamWaitEvent(A, filled[3 * i + 2], 0); // join
+    return st;
+}
+
 extern "C" {
 
 int gr_level_rk3_step(gr_level* L, const gr_ccz4_params* prm, double dt, int precision, int bc_mode, int pre_fill,
@@ -757,6 +814,58 @@ int gr_level_rk3_step(gr_level* L, const gr_ccz4_params* prm, double dt, int pre
         L->have_graph = true;
     }
     GR_HIP(e, hipGraphLaunch(L->exec, e->stream));
+    return GR_OK;
+}
+
+int gr_levels_rk3_step(int n, gr_level* const* levels, const double* dts, const gr_ccz4_params* prm, int precision,
+                       int bc_mode, int pre_fill, int use_graph) {
+    if (n < 1 || !levels || !dts || !levels[0]) return GR_ERR_ARG;
+    gr_engine* e = levels[0]->eng;
+    int rc = use_device(e);
+    if (rc) return rc;
+    if ((rc = check_params(e, prm))) return rc;
+    for (int i = 0; i < n; ++i) {
+        if (!levels[i] || levels[i]->eng != e) return fail(e, GR_ERR_ARG, "every level must belong to one engine");
+        for (int j = 0; j < i; ++j)
+            if (levels[j] == levels[i]) return fail(e, GR_ERR_ARG, "a level appears twice");
+        if (!std::isfinite(dts[i])) return fail(e, GR_ERR_ARG, "dt must be finite");
+        if (levels[i]->profile) return fail(e, GR_ERR_STATE, "profiled levels step one at a time");
+    }
+    if (precision != GR_PREC_FP64 && precision != GR_PREC_MIXED) return fail(e, GR_ERR_ARG, "unknown precision");
+    if (bc_mode < 0 || bc_mode > 2) return fail(e, GR_ERR_ARG, "bc_mode must be 0, 1 or 2");
+    if (pre_fill < -1 || pre_fill > 2) return fail(e, GR_ERR_ARG, "pre_fill must be -1, 0, 1 or 2");
+    if (!e->fill_stream) GR_HIP(e, hipStreamCreate(&e->fill_stream)); // default priority
+    if (!use_graph) {
+        GR_HIP(e, enqueue_levels_rk3(e, n, levels, dts, prm, bc_mode, pre_fill, precision));
+        return GR_OK;
+    }
+    std::vector<double> key = {(double)n, prm->kappa1, prm->kappa2, prm->eta, prm->ko_sigma, prm->chi_blend_center,
+                               prm->chi_blend_width, (double)precision, (double)bc_mode, (double)pre_fill};
+    for (int i = 0; i < n; ++i) {
+        key.push_back((double)reinterpret_cast<uintptr_t>(levels[i]));
+        key.push_back((double)levels[i]->generation);
+        key.push_back(dts[i]);
+    }
+    if (!e->ls_exec || key != e->ls_key) {
+        if (e->ls_exec) GR_HIP(e, hipGraphExecDestroy(e->ls_exec));
+        if (e->ls_graph) GR_HIP(e, hipGraphDestroy(e->ls_graph));
+        e->ls_exec = nullptr;
+        e->ls_graph = nullptr;
+        e->ls_key.clear();
+        GR_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t st = enqueue_levels_rk3(e, n, levels, dts, prm, bc_mode, pre_fill, precision);
+        hipGraph_t g = nullptr;
+        hipError_t st2 = hipStreamEndCapture(e->stream, &g);
+        if (st != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hipfail(e, st, "capture of the level-set step");
+        }
+        if (st2 != hipSuccess) return hipfail(e, st2, "hipStreamEndCapture");
+        e->ls_graph = g;
+        GR_HIP(e, hipGraphInstantiate(&e->ls_exec, e->ls_graph, nullptr, nullptr, 0));
+        e->ls_key = key;
+    }
+    GR_HIP(e, hipGraphLaunch(e->ls_exec, e->stream));
     return GR_OK;
 }
 
@@ -1074,7 +1183,7 @@ int gr_level_autotune(gr_level* L, const gr_ccz4_params* prm, double dt, double*
         int best = 0;
         double tbest = INFINITY;
         for (int v = 0; v < gr::FP64_STAGE_VARIANTS; ++v) {
-            if (v == 1 && !G.batch.pairs_ok) {
+            if (v != 0 && !G.batch.pairs_ok) {
                 out.push_back(-1.0);
                 continue;
             }
@@ -1123,7 +1232,7 @@ int gr_level_get_tuning(gr_level* L, int kind, int group, int* value) {
     if (kind == GR_TUNE_STAGE) {
         if (group < 0 || group >= (int)L->groups.size()) return fail(e, GR_ERR_ARG, "group index out of range");
         const gr::Batch& B = L->groups[group].batch;
-        *value = (B.variant == 1 && B.pairs_ok) ? 1 : 0;
+        *value = (B.variant != 0 && B.pairs_ok) ? B.variant : 0;
         return GR_OK;
     }
     if (kind == GR_TUNE_GHOST) {
@@ -1139,7 +1248,7 @@ int gr_level_set_tuning(gr_level* L, int kind, int group, int value) {
     if (kind == GR_TUNE_STAGE) {
         if (group < 0 || group >= (int)L->groups.size()) return fail(e, GR_ERR_ARG, "group index out of range");
         if (!tune_valid(kind, value)) return fail(e, GR_ERR_ARG, "unknown staging variant");
-        if (value == 1 && !L->groups[group].batch.pairs_ok)
+        if (value != 0 && !L->groups[group].batch.pairs_ok)
             return fail(e, GR_ERR_ARG, "direct LDS staging needs even row lengths and 16-byte aligned buffers");
         L->groups[group].batch.variant = value;
     } else if (kind == GR_TUNE_GHOST) {

@@ -372,6 +372,8 @@ struct TileIO {
         double val = combine_exact(cf.a1, unv[v], cf.a2, uin, cf.a3dt, rr);
         if (v == CHI) val = floor_chi(val);
         if (v == ALPHA) val = floor_alpha(val);
+        // (one wave per SIMD: the held metric costs no occupancy here, and reading it back at
+        // the end would expose a load latency)
         if (MODE == MODE_FINAL && v >= GXX && v < GXX + 6) {
             hg[v - GXX] = val;
             return;
@@ -429,7 +431,12 @@ struct TileIODma {
     unsigned doff[KW]; // byte offset of this lane's pair in each of its wave's chunks
     double unb;
     double unv[NV];
-    double hg[6], hA[6];
+    double hA[6];
+    // the barrier that starts a field waits for every outstanding vector memory operation
+    // (it must, for the direct LDS loads), so a field's result is stored after the next
+    // barrier instead of just before it: the store then has a whole field to complete
+    double pend_val;
+    int pend_v;
 
     __device__ void init(int tid, int ox, int oy, int oz, int nt0, int nt1, int nt2) {
         wave = __builtin_amdgcn_readfirstlane(tid / 64); // uniform: LDS bases go to M0 from scalars
@@ -444,7 +451,13 @@ struct TileIODma {
             const bool ok = c < NCH && pl < T::BZ && r < T::BX * T::BY && gx + 1 < nt0 && gy < nt1 && gz < nt2;
             doff[k] = ok ? 8u * (unsigned)(gx + nt0 * (gy + nt1 * gz)) : 0u;
         }
+        pend_v = -1;
         prefetch(0);
+    }
+
+    __device__ __forceinline__ void flush() {
+        if (pend_v >= 0 && active) gstore(out + pend_v * stride, cb, pend_val);
+        pend_v = -1;
     }
 
     __device__ __forceinline__ void prefetch(int pos) {
@@ -477,6 +490,7 @@ struct TileIODma {
     __device__ const double* field(int v) {
         const int pos = field_pos(v);
         __syncthreads(); // field pos has landed in its buffer; every thread is done with pos - 1
+        flush();
         if (MODE != MODE_RHS) unv[v] = unb;
         prefetch(pos + 1);
         return lds + (pos & 1) * LBUF + cl;
@@ -485,26 +499,32 @@ struct TileIODma {
     // ---- Sink
     __device__ __forceinline__ void put(int v, double r, double ko, double uin) {
         const double rr = ko_on ? r + ko : r;
+        flush(); // at most one result waits (several per field only after the last barrier)
         if (MODE == MODE_RHS) {
-            if (active) gstore(out + v * stride, cb, rr);
+            pend_v = v;
+            pend_val = rr;
             return;
         }
         double val = combine_exact(cf.a1, unv[v], cf.a2, uin, cf.a3dt, rr);
         if (v == CHI) val = floor_chi(val);
         if (v == ALPHA) val = floor_alpha(val);
-        if (MODE == MODE_FINAL && v >= GXX && v < GXX + 6) {
-            hg[v - GXX] = val;
-            return;
-        }
+        // last stage: the new metric is stored now and read back for the algebraic
+        // constraints at the end (the output is the state, whose metric nothing reads after
+        // this cell's own combine), instead of being held in registers across the kernel
         if (MODE == MODE_FINAL && v >= AXX && v < AXX + 6) {
             hA[v - AXX] = val;
             return;
         }
-        if (active) gstore(out + v * stride, cb, val);
+        pend_v = v;
+        pend_val = val;
     }
 
     __device__ __forceinline__ void finish() {
+        flush();
         if (MODE != MODE_FINAL) return;
+        double hg[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) hg[a] = gload(out + (GXX + a) * stride, cb);
         algebraic_exact(hg, hA);
         if (!active) return;
 #pragma unroll
@@ -569,7 +589,7 @@ struct TileIO32 {
         const int slot = pos % PF;
         const gdouble* f = in + v * stride;
 #pragma unroll
-        for (int i = 0; i < NL; ++i) buf[slot][i] = gload_sv(f, goff[i]);
+        for (int i = 0; i < NL; ++i) buf[slot][i] = gload(f, goff[i]);
         refb[slot] = f[ridx];
         inb[slot] = gload(f, cb);
         if (MODE != MODE_RHS) unb[slot] = gload(un + v * stride, cb);
@@ -739,23 +759,28 @@ __device__ __forceinline__ void load_sources(const Sources& S, long cidx, CellSo
     for (int a = 0; a < 6; ++a) src.Sij[a] = S.Sij[6 * cidx + a];
 }
 
-// part `part` of a tile: TX x TY x TZ cells at x = (part % GR_FP64_SPLIT) * TX,
-// z = (part / GR_FP64_SPLIT) * TZ inside the tile
+// staging variant SV: 0 = registers into a padded box (layout 0); 1, 2 = direct LDS loads
+// into an unpadded box (layout 1); 2 runs parts twice as deep in z
+constexpr int sv_layout(int SV) { return SV == 0 ? 0 : 1; }
+
+// part `part` of a tile: TX x TY x TZ cells at x = (part % (TILE_X / TX)) * TX,
+// z = (part / (TILE_X / TX)) * TZ inside the tile
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT, int SV>
 __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R, int part, double* lds) {
-    using T = TileGeom<TX, TY, TZ, true, 8, SV>;
+    using T = TileGeom<TX, TY, TZ, true, 8, sv_layout(SV)>;
+    constexpr int XS = GR_TILE_X / TX;
     const BlockDesc& b = R.b;
     const int tid = threadIdx.x;
     int lx = 0, ly = 0, lz = 0;
     T::Lanes::cell(tid, lx, ly, lz);
-    const int x0 = R.tix * GR_TILE_X + (part % GR_FP64_SPLIT) * TX;
-    const int z0 = R.tiz * GR_TILE_Z + (part / GR_FP64_SPLIT) * TZ;
+    const int x0 = R.tix * GR_TILE_X + (part % XS) * TX;
+    const int z0 = R.tiz * GR_TILE_Z + (part / XS) * TZ;
     const int ix = x0 + lx, iy = R.tiy * TY + ly, iz = z0 + lz; // interior-relative
     const bool active = ix < b.n[0] && iy < b.n[1] && iz < b.n[2];
     const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
     const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
 
-    typename std::conditional<SV == 1, TileIODma<TX, TY, TZ, MODE>, TileIO<TX, TY, TZ, MODE, GR_FP64_PREFETCH>>::type io;
+    typename std::conditional<SV != 0, TileIODma<TX, TY, TZ, MODE>, TileIO<TX, TY, TZ, MODE, GR_FP64_PREFETCH>>::type io;
     io.lds = lds;
     io.in = R.in;
     io.un = R.un;
@@ -774,28 +799,30 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
     ccz4_cell<double, HAS_SRC, KO_EXACT>(io, io, A.K, A.P, src);
 }
 
-// parts of a tile in the FP64 kernels
-constexpr int FP64_PARTS = GR_FP64_SPLIT * GR_FP64_ZSPLIT;
+// parts of a tile of TX x . x TZ cells
+template <int TX, int TZ>
+constexpr int parts_of() { return (GR_TILE_X / TX) * (GR_TILE_Z / TZ); }
 
 // LDS doubles of one FP64 part's two field buffers under staging variant SV
 template <int TX, int TY, int TZ, int SV>
 constexpr int fp64_lds_doubles() {
-    return SV == 1 ? 2 * TileIODma<TX, TY, TZ, MODE_RHS>::LBUF : 2 * (TileGeom<TX, TY, TZ, true, 8, 0>::NLDS + 1);
+    return SV != 0 ? 2 * TileIODma<TX, TY, TZ, MODE_RHS>::LBUF : 2 * (TileGeom<TX, TY, TZ, true, 8, 0>::NLDS + 1);
 }
 
 // whether part `part` of tile R holds no cell of the block (uniform over the workgroup)
 template <int TX, int TZ>
 __device__ __forceinline__ bool part_outside(const TileRef& R, int part) {
-    return R.tix * GR_TILE_X + (part % GR_FP64_SPLIT) * TX >= R.b.n[0] ||
-        R.tiz * GR_TILE_Z + (part / GR_FP64_SPLIT) * TZ >= R.b.n[2];
+    constexpr int XS = GR_TILE_X / TX;
+    return R.tix * GR_TILE_X + (part % XS) * TX >= R.b.n[0] || R.tiz * GR_TILE_Z + (part / XS) * TZ >= R.b.n[2];
 }
 
-// one part of one tile per workgroup (FP64_PARTS consecutive workgroups per tile)
+// one part of one tile per workgroup (parts_of consecutive workgroups per tile)
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT, int SV>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
+    constexpr int NP = parts_of<TX, TZ>();
     __shared__ double lds[fp64_lds_doubles<TX, TY, TZ, SV>()];
-    const int slot = tile_of_block(blockIdx.x, A.ntiles * FP64_PARTS, A.xcd_remap);
-    const int tile = slot / FP64_PARTS, part = slot % FP64_PARTS;
+    const int slot = tile_of_block(blockIdx.x, A.ntiles * NP, A.xcd_remap);
+    const int tile = slot / NP, part = slot % NP;
     if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
     const TileRef R = resolve_tile(A, tile);
     if (part_outside<TX, TZ>(R, part)) return;
@@ -822,7 +849,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_sc
         __syncthreads();
         for (int item = 0; item < n; ++item) {
             const TileRef R = resolve_tile(A, list[item]);
-            for (int part = 0; part < FP64_PARTS; ++part) {
+            for (int part = 0; part < parts_of<TX, TZ>(); ++part) {
                 if (part_outside<TX, TZ>(R, part)) continue; // uniform over the workgroup
                 stage_tile<TX, TY, TZ, MODE, HAS_SRC, KO_EXACT, SV>(A, R, part, lds);
                 __syncthreads(); // the next part reuses the LDS buffers
@@ -841,6 +868,14 @@ static_assert(!GR_LDS_PAD || FP64_TX != 8 || TILE_X != 16 || TILE_Y != 4 ||
                    TileGeom<FP64_TX, FP64_TY, FP64_TZ>::BX == 24 && TileGeom<TILE_X, TILE_Y, TILE_Z, true, 4>::BX == 48),
               "LDS layouts of the default tiles: FP64 rows of 24 (layout 0), rows of 16 in planes of 200 (layout "
               "1); FP32 rows of 48");
+// part shapes of the staging variants: 0 and 1 the build's part, 2 twice as deep in z
+template <int SV>
+struct SvShape {
+    static constexpr int TX = SV == 0 ? FP64_TX : 8, TY = SV == 0 ? FP64_TY : 4, TZ = SV == 0 ? FP64_TZ : (SV == 1 ? 4 : 8);
+    static constexpr int NT = TX * TY * TZ, PARTS = parts_of<TX, TZ>();
+};
+static_assert(TILE_X == 16 && TILE_Y == 4 && TILE_Z == 8, "the direct-staging parts (8x4x4, 8x4x8) tile 16x4x8 tiles");
+static_assert(FP64_STAGE_VARIANTS == 3, "staging variants 0, 1, 2");
 // workgroups of the FP64 pass of the mixed form: 256 threads per CU
 constexpr int FP64_SCAN_GRID = 256 * 256 / (FP64_TX * FP64_TY * FP64_TZ);
 
@@ -901,22 +936,24 @@ __global__ __launch_bounds__(TX* TY* TZ) void classify_tiles_kernel(StageArgs A,
     if (tid == 0) flags[tile] = (unsigned char)bad;
 }
 
+// scan: nblk workgroups gather the flagged tiles; otherwise one workgroup per part
 template <int MODE, bool HAS_SRC, int SV>
-hipError_t launch_stage_sv(const StageArgs& A, int nblk, hipStream_t s) {
+hipError_t launch_stage_sv(const StageArgs& A, int scan_grid, hipStream_t s) {
+    using S = SvShape<SV>;
     if (A.scan)
-        stage_kernel_scan<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0, SV>
-            <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
-    else
-        stage_kernel<FP64_TX, FP64_TY, FP64_TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0, SV>
-            <<<dim3(nblk), dim3(FP64_TX * FP64_TY * FP64_TZ), 0, s>>>(A);
+        stage_kernel_scan<S::TX, S::TY, S::TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0, SV>
+            <<<dim3(scan_grid), dim3(S::NT), 0, s>>>(A);
+    else if (A.ntiles > 0)
+        stage_kernel<S::TX, S::TY, S::TZ, MODE, HAS_SRC, GR_FP64_KO_EXACT != 0, SV>
+            <<<dim3(A.ntiles * S::PARTS), dim3(S::NT), 0, s>>>(A);
     return hipGetLastError();
 }
 
 template <int MODE, bool HAS_SRC>
-hipError_t launch_stage_t(const StageArgs& A, int nblk, int variant, hipStream_t s) {
-    if (nblk < 1) return hipSuccess;
-    if (variant == 1) return launch_stage_sv<MODE, HAS_SRC, 1>(A, nblk, s);
-    return launch_stage_sv<MODE, HAS_SRC, 0>(A, nblk, s);
+hipError_t launch_stage_t(const StageArgs& A, int scan_grid, int variant, hipStream_t s) {
+    if (variant == 1) return launch_stage_sv<MODE, HAS_SRC, 1>(A, scan_grid, s);
+    if (variant == 2) return launch_stage_sv<MODE, HAS_SRC, 2>(A, scan_grid, s);
+    return launch_stage_sv<MODE, HAS_SRC, 0>(A, scan_grid, s);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1793,11 +1830,11 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], bool has_src, con
                              double ko_sigma, int mode, const StageCoef& cf, hipStream_t s,
                              const unsigned char* flags) {
     // the direct LDS staging reads 16-byte pairs: other batches run the register staging
-    const int variant = (B.variant == 1 && B.pairs_ok) ? 1 : 0;
+    const int variant = (B.variant != 0 && B.pairs_ok) ? B.variant : 0;
     StageArgs A = stage_args(B, sel, P, ko_sigma, cf, flags, 1);
     // with flags (the FP64 tiles of the mixed form: few, near the puncture) a small grid
-    // gathers them instead of launching one workgroup per tile of the level
-    int nblk = A.ntiles * FP64_PARTS;
+    // gathers them instead of launching one workgroup per part of every tile of the level
+    int nblk = 0;
     if (flags) {
         A.scan = 1;
         nblk = A.ntiles < FP64_SCAN_GRID ? A.ntiles : FP64_SCAN_GRID;

@@ -52,9 +52,10 @@ constexpr int GHOST_WG = 256;
 // block `one`. sel_* pick the buffer of
 // each block that is the stage input, u^n and the output (0, 1, 2 = buf[0..2]).
 // FP64 staging variants of the stage kernel (chosen per launch, by the autotuner): 0 = halo
-// prefetched into registers and written to a padded LDS box (one wave per SIMD); 1 = halo
-// loaded straight into an unpadded LDS box (two waves per SIMD), for even row lengths
-constexpr int FP64_STAGE_VARIANTS = 2;
+// prefetched into registers and written to a padded LDS box (parts of 8x4x4 cells, one wave
+// per SIMD); 1 = halo loaded straight into an unpadded LDS box (8x4x4, two waves per SIMD);
+// 2 = as 1 with parts of 8x4x8 (less halo per cell). 1 and 2 need even row lengths.
+constexpr int FP64_STAGE_VARIANTS = 3;
 
 struct Batch {
     const BatchBlock* table;

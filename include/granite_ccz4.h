@@ -174,6 +174,17 @@ int gr_level_rk3_stage(gr_level* lvl, const gr_ccz4_params* prm, double dt, int 
  * finite) run the FP64 stage kernel; all other tiles the FP32 one. Default 0.2. */
 int gr_level_set_mixed_policy(gr_level* lvl, double chi_fp64_below);
 
+/* One SSP-RK3 step of n independent levels of one engine, level i with dts[i] (the batched
+ * uneven-block workload: every refinement level advanced by its own step): the same
+ * launches as gr_level_rk3_step on each level, scheduled over two streams so that each
+ * level's ghost fill (memory-bound) overlaps the following levels' stage kernels
+ * (compute-bound); each level's own stages and fills keep their order (events). Results are
+ * bit-identical to stepping the levels one by one. use_graph: the whole schedule as one
+ * captured HIP graph, re-captured when the level set, a dt, the parameters or a level's
+ * tuning change. */
+int gr_levels_rk3_step(int n, gr_level* const* levels, const double* dts, const gr_ccz4_params* prm, int precision,
+                       int bc_mode, int pre_fill, int use_graph);
+
 /* RHS (with dissipation) of block `block`'s current state in the given precision, written
  * to host_rhs (22*stride doubles, interior cells; ghost cells set to 0). The FP64 form is
  * CCZ4Evolution::computeRHSVacuum (src/spacetime/ccz4.cpp:229-245) on device-resident data;
@@ -313,8 +324,9 @@ int gr_level_profile_read(gr_level* lvl, int kind, double* total_ms, int* launch
  * collapse(3) schedule(static), src/spacetime/ccz4.cpp:266-268) ---------------------------- */
 
 /* Tuned choices. GR_TUNE_STAGE: the FP64 stage kernel's halo staging, 0 = prefetch through
- * registers into a padded LDS box (one wave per SIMD), 1 = direct global-to-LDS loads into an
- * unpadded box (two waves per SIMD; blocks with an even row length only, others run 0).
+ * registers into a padded LDS box (8x4x4-cell parts, one wave per SIMD), 1 = direct
+ * global-to-LDS loads into an unpadded box (8x4x4, two waves per SIMD), 2 = as 1 with
+ * 8x4x8-cell parts (less halo per cell); 1 and 2 need even row lengths (others run 0).
  * GR_TUNE_GHOST / GR_TUNE_TRANSFER: field groups (1, 2, 4 or 11) of the ghost fill and of the
  * prolongation and restriction launches. */
 enum gr_tune_kind { GR_TUNE_STAGE = 0, GR_TUNE_GHOST = 1, GR_TUNE_TRANSFER = 2 };
@@ -324,7 +336,7 @@ enum gr_tune_kind { GR_TUNE_STAGE = 0, GR_TUNE_GHOST = 1, GR_TUNE_TRANSFER = 2 }
  * on this level's own blocks, keeps the fastest in the level and in the process-wide tuning
  * table under the level's block-size class (mean interior cells per block: < 24^3, < 48^3,
  * < 96^3, larger). times (optional, cap entries) receives the measured ms per launch in the
- * order: for each group, each stage variant (-1 where not applicable); then each ghost group
+ * order: for each group, each stage variant 0, 1, 2 (-1 where not applicable); then each ghost group
  * count (1, 2, 4, 11); *n = entries written. Captured graphs of the level are re-recorded. */
 int gr_level_autotune(gr_level* lvl, const gr_ccz4_params* prm, double dt, double* times, int cap, int* n);
 /* gr_level_autotune of every level (dt0 / 2^l), then the transfer field groups of every

@@ -32,7 +32,7 @@ def bl_case(oracle):
     return g, u
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_stage_variants_against_oracle(engine, oracle, variant):
     import granite_amd as ga
 
@@ -66,8 +66,9 @@ def test_direct_staging_refused_for_odd_rows(engine, oracle):
     g = Geom((13, 8, 8), 4, (-1.6, -1.0, -1.0), (1.65, 1.0, 1.0))
     blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
     lvl = ga.Level([blk], engine)
-    with pytest.raises(ga.EngineError):
-        lvl.set_tuning(ga.TUNE_STAGE, 0, 1)
+    for v in (1, 2):
+        with pytest.raises(ga.EngineError):
+            lvl.set_tuning(ga.TUNE_STAGE, 0, v)
     assert lvl.tuning(ga.TUNE_STAGE, 0) == 0
     lvl.close()
 
@@ -149,7 +150,8 @@ def test_autotune_choice_is_a_launch_form(engine, oracle, tmp_path):
     lvl.upload()
     say("autotune a level")
     times = lvl.autotune(p, 0.0625)
-    assert len(times["stage"]) == 1 and all(t is not None and t > 0 for t in times["stage"][0])
+    assert len(times["stage"]) == 1 and len(times["stage"][0]) == 3
+    assert all(t is not None and t > 0 for t in times["stage"][0])
     assert sorted(times["ghost"]) == [1, 2, 4, 11] and all(t > 0 for t in times["ghost"].values())
     sv, gg = lvl.tuning(ga.TUNE_STAGE, 0), lvl.tuning(ga.TUNE_GHOST, 0)
     st = times["stage"][0]

@@ -32,7 +32,7 @@ def test_table_round_trip(tmp_path):
     assert [ln for ln in open(out) if not ln.startswith("#")] == []
 
 
-@pytest.mark.parametrize("bad", ["0 3 0 2\n", "1 2 0 3\n", "3 0 0 1\n", "0 4 0 1\n", "0 1 2 0\n", "0 1 0\n",
+@pytest.mark.parametrize("bad", ["0 3 0 3\n", "1 2 0 3\n", "3 0 0 1\n", "0 4 0 1\n", "0 1 2 0\n", "0 1 0\n",
                                  "0 1 0 1 9\n", "x 1 0 1\n"])
 def test_bad_entries_rejected(tmp_path, bad):
     ga, _ = _lib()
