@@ -138,8 +138,8 @@ def run_batched(ga, eng, p, args):
         tuning = []
         for lv, dx in levels:
             lv.init_brill_lindquist(masses, pos)
-            if tune and not args.no_tune:
-                tuning.append(level_tuning(ga, lv, lv.autotune(p, 0.25 * dx)))
+            if tune:
+                tuning.append(level_tuning(ga, lv, lv.autotune(p, 0.25 * dx) if args.autotune else None))
         def step():
             if together:
                 # the level set as one schedule: each level's ghost fills overlap the
@@ -192,7 +192,7 @@ def run_batched(ga, eng, p, args):
     return out
 
 
-def run_noisy_192(ga, eng, p, steps, tune=True):
+def run_noisy_192(ga, eng, p, steps, tune=True, autotune=True):
     """The mixed-precision single-puncture case: BL M = 1, one 192^3 block on [-16,16]^3
     (dx = 1/6, dt = 1/24) with N(0, 1e-8^2) added to the conformal metric and K (seed
     20240611, the engine's portable generator). Per-field max RHS error of the mixed form
@@ -212,7 +212,7 @@ def run_noisy_192(ga, eng, p, steps, tune=True):
         lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
         lvl.add_noise(20240611, 1e-8, noise_fields)
         if tune:
-            out.setdefault("tuning", level_tuning(ga, lvl, lvl.autotune(p, dt)))
+            out.setdefault("tuning", level_tuning(ga, lvl, lvl.autotune(p, dt) if autotune else None))
         r = lvl.rhs(0, p, prec)[:, 4:-4, 4:-4, 4:-4]
         res[name] = r
         st = dict(precision=prec, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
@@ -288,8 +288,9 @@ def run_hierarchy(ga, eng, p, args):
         for l in range(1, len(levels)):
             H.prolongate(l)
         tuning = None
-        if not args.no_tune:
-            H.autotune(p, dt0)
+        if args.autotune or args.tune_table:
+            if args.autotune:
+                H.autotune(p, dt0)
             tuning = {"per_level": [level_tuning(ga, lv) for lv in levels],
                       "transfer_field_groups": [H.tuning(l) for l in range(1, len(levels))]}
         H.fill_ghosts(0)
@@ -333,6 +334,9 @@ def main():
     ap.add_argument("--no-hierarchy", action="store_true", help="skip the binary-puncture hierarchy section")
     ap.add_argument("--no-noisy", action="store_true", help="skip the 192^3 noisy mixed-precision section")
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel variants (no autotuning)")
+    ap.add_argument("--tune-table", metavar="PATH",
+                    help="load this tuning table (written by --tune-save) instead of autotuning")
+    ap.add_argument("--tune-save", metavar="PATH", help="write the tuning table in force at the end of the run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -346,6 +350,10 @@ def main():
 
     import granite_amd as ga
 
+    # variants: measured here (autotune), taken from a saved table, or the defaults
+    args.autotune = not args.no_tune and not args.tune_table
+    if args.tune_table:
+        ga.tune_load(args.tune_table)
     eng = ga.Engine(0)
     p = ga.CCZ4Params(kappa1=0.02, kappa2=0.0, eta=2.0, ko_sigma=0.1)
     cells = N ** 3
@@ -354,7 +362,9 @@ def main():
         blk = ga.GridBlock(0, 0, (N, N, N), (-HALF,) * 3, (HALF,) * 3, 4)
         lvl = ga.Level([blk], eng)
         lvl.init_brill_lindquist([1.0], [[0.0, 0.0, 0.0]])
-        tuning = None if args.no_tune else level_tuning(ga, lvl, lvl.autotune(p, DT))
+        tuning = None
+        if args.autotune or args.tune_table:
+            tuning = level_tuning(ga, lvl, lvl.autotune(p, DT) if args.autotune else None)
         step = dict(precision=precision, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
         for _ in range(args.warmup):
             lvl.rk3_step(p, DT, use_graph=True, **step)
@@ -394,7 +404,8 @@ def main():
     mix = run_form(ga.PREC_MIXED)
     batched = None if args.no_batched else run_batched(ga, eng, p, args)
     hier = None if args.no_hierarchy else run_hierarchy(ga, eng, p, args)
-    noisy = None if args.no_noisy else run_noisy_192(ga, eng, p, 100, tune=not args.no_tune)
+    noisy = None if args.no_noisy else run_noisy_192(ga, eng, p, 100, tune=args.autotune or bool(args.tune_table),
+                                                          autotune=args.autotune)
     value = f64["value"]
     ms_per_step = f64["ms_per_step"]
 
@@ -498,6 +509,8 @@ def main():
     }
     if cpu.get("value"):
         line["gpu_over_cpu"] = value / cpu["value"]
+    if args.tune_save:
+        ga.tune_save(args.tune_save)
     print(json.dumps(line), flush=True)
     eng.close()
 

@@ -40,6 +40,10 @@
 #ifndef GR_FP64_MIN_WAVES
 #define GR_FP64_MIN_WAVES 1
 #endif
+// parts of a tile along x the mixed kernel runs as separate workgroups (1 or 2)
+#ifndef GR_MIXED_SPLIT
+#define GR_MIXED_SPLIT 1
+#endif
 #ifndef GR_MIXED_MIN_WAVES
 #define GR_MIXED_MIN_WAVES 1
 #endif
@@ -928,18 +932,25 @@ static_assert(FP64_STAGE_VARIANTS == 3, "staging variants 0, 1, 2");
 // workgroups of the FP64 pass of the mixed form: 256 threads per CU
 constexpr int FP64_SCAN_GRID = 256 * 256 / (FP64_TX * FP64_TY * FP64_TZ);
 
+// one part of TX x TY x TZ cells (TILE_X / TX parts along x) of a launch tile per workgroup;
+// the tile is the unit of the FP32 / FP64 choice
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC>
 __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_mixed(StageArgs A) {
     using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0, 4>;
+    constexpr int NP = GR_TILE_X / TX;
+    static_assert(GR_TILE_X % TX == 0 && TY == GR_TILE_Y && TZ == GR_TILE_Z, "parts split the tile along x");
     __shared__ float lds[2 * (T::NLDS + 1)];
 
-    const int tile = tile_of_block(blockIdx.x, A.ntiles, A.xcd_remap);
+    const int slot = tile_of_block(blockIdx.x, A.ntiles * NP, A.xcd_remap);
+    const int tile = slot / NP, part = slot % NP;
     if (A.flags && A.flags[tile] != A.want) return;
     const TileRef R = resolve_tile(A, tile);
     const BlockDesc& b = R.b;
+    const int x0 = R.tix * GR_TILE_X + part * TX;
+    if (x0 >= b.n[0]) return; // whole workgroup, before any barrier
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
-    const int ix = R.tix * TX + lx, iy = R.tiy * TY + ly, iz = R.tiz * TZ + lz;
+    const int ix = x0 + lx, iy = R.tiy * TY + ly, iz = R.tiz * TZ + lz;
     const bool active = ix < b.n[0] && iy < b.n[1] && iz < b.n[2];
     const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
     const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
@@ -952,12 +963,12 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_m
     io.stride = b.stride;
     io.cidx = cidx;
     io.cb = 8u * (unsigned)cidx;
-    io.ridx = (b.ng + R.tix * TX) + (long)b.nt[0] * ((b.ng + R.tiy * TY) + (long)b.nt[1] * (b.ng + R.tiz * TZ));
+    io.ridx = (b.ng + x0) + (long)b.nt[0] * ((b.ng + R.tiy * TY) + (long)b.nt[1] * (b.ng + R.tiz * TZ));
     io.cl = T::lds(lx, ly, lz);
     io.active = active;
     io.ko_on = A.ko_on != 0;
     io.cf = A.cf;
-    io.init(tid, b.ng + R.tix * TX, b.ng + R.tiy * TY, b.ng + R.tiz * TZ, b.nt[0], b.nt[1], b.nt[2]);
+    io.init(tid, b.ng + x0, b.ng + R.tiy * TY, b.ng + R.tiz * TZ, b.nt[0], b.nt[1], b.nt[2]);
 
     CellSources src{};
     if (HAS_SRC) load_sources(R.src, cidx, src);
@@ -1900,8 +1911,9 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], bool has_src, con
 
 template <int MODE, bool HAS_SRC>
 static hipError_t launch_mixed_t(const StageArgs& A, hipStream_t s) {
-    stage_kernel_mixed<TILE_X, TILE_Y, TILE_Z, MODE, HAS_SRC>
-        <<<dim3(A.ntiles), dim3(TILE_X * TILE_Y * TILE_Z), 0, s>>>(A);
+    constexpr int PX = TILE_X / GR_MIXED_SPLIT;
+    stage_kernel_mixed<PX, TILE_Y, TILE_Z, MODE, HAS_SRC>
+        <<<dim3(A.ntiles * GR_MIXED_SPLIT), dim3(PX * TILE_Y * TILE_Z), 0, s>>>(A);
     return hipGetLastError();
 }
 
