@@ -420,8 +420,14 @@ struct TileIODma {
     static constexpr int NW = T::NT / 64; // waves
     static constexpr int KW = (NCH + NW - 1) / NW; // chunk loads per wave
     static_assert(T::NT % 64 == 0, "whole waves");
+    // two field buffers; in the last stage also the new metric of every cell, from its
+    // combine to the algebraic constraints at the end, where that still lets two waves per
+    // SIMD (512 / NT workgroups) share the CU's 160 KB; otherwise the metric is stored at
+    // once and read back at the end
+    static constexpr bool METRIC_LDS = MODE == MODE_FINAL && (2 * LBUF + 6 * T::NT) * 8 * (512 / T::NT) <= 163840;
+    static constexpr int LDS_DOUBLES = 2 * LBUF + (METRIC_LDS ? 6 * T::NT : 0);
 
-    double* lds; // 2 * LBUF doubles
+    double* lds; // LDS_DOUBLES doubles
     const gdouble* in;
     const gdouble* un;
     gdouble* out;
@@ -434,6 +440,7 @@ struct TileIODma {
     bool ko_on;
     StageCoef cf;
     unsigned doff[KW]; // byte offset of this lane's pair in each of its wave's chunks
+    int tid;
     double unb;
     double unv[NV];
     double hA[6];
@@ -466,7 +473,8 @@ struct TileIODma {
     }
     static_assert(pair_read_matches_boxes(), "fetched rows = rows some stencil reads");
 
-    __device__ void init(int tid, int ox, int oy, int oz, int nt0, int nt1, int nt2) {
+    __device__ void init(int tid_, int ox, int oy, int oz, int nt0, int nt1, int nt2) {
+        tid = tid_;
         wave = __builtin_amdgcn_readfirstlane(tid / 64); // uniform: LDS bases go to M0 from scalars
         const int lane = tid % 64;
 #pragma unroll
@@ -550,9 +558,13 @@ struct TileIODma {
         double val = combine_exact(cf.a1, unv[v], cf.a2, uin, cf.a3dt, rr);
         if (v == CHI) val = floor_chi(val);
         if (v == ALPHA) val = floor_alpha(val);
-        // last stage: the new metric is stored now and read back for the algebraic
-        // constraints at the end (the output is the state, whose metric nothing reads after
-        // this cell's own combine), instead of being held in registers across the kernel
+        // last stage: the new metric waits in LDS (this thread's own slots; or in the output,
+        // read back at the end) for the algebraic constraints, instead of in registers across
+        // the rest of the RHS; A~ comes last and is held in registers
+        if (METRIC_LDS && v >= GXX && v < GXX + 6) {
+            lds[2 * LBUF + (v - GXX) * T::NT + tid] = val;
+            return;
+        }
         if (MODE == MODE_FINAL && v >= AXX && v < AXX + 6) {
             hA[v - AXX] = val;
             return;
@@ -566,7 +578,7 @@ struct TileIODma {
         if (MODE != MODE_FINAL) return;
         double hg[6];
 #pragma unroll
-        for (int a = 0; a < 6; ++a) hg[a] = gload(out + (GXX + a) * stride, cb);
+        for (int a = 0; a < 6; ++a) hg[a] = METRIC_LDS ? lds[2 * LBUF + a * T::NT + tid] : gload(out + (GXX + a) * stride, cb);
         algebraic_exact(hg, hA);
         if (!active) return;
 #pragma unroll
@@ -857,9 +869,9 @@ template <int TX, int TZ>
 constexpr int parts_of() { return (GR_TILE_X / TX) * (GR_TILE_Z / TZ); }
 
 // LDS doubles of one FP64 part's two field buffers under staging variant SV
-template <int TX, int TY, int TZ, int SV>
+template <int TX, int TY, int TZ, int SV, int MODE>
 constexpr int fp64_lds_doubles() {
-    return SV != 0 ? 2 * TileIODma<TX, TY, TZ, MODE_RHS>::LBUF : 2 * (TileGeom<TX, TY, TZ, true, 8, 0>::NLDS + 1);
+    return SV != 0 ? TileIODma<TX, TY, TZ, MODE>::LDS_DOUBLES : 2 * (TileGeom<TX, TY, TZ, true, 8, 0>::NLDS + 1);
 }
 
 // whether part `part` of tile R holds no cell of the block (uniform over the workgroup)
@@ -873,7 +885,7 @@ __device__ __forceinline__ bool part_outside(const TileRef& R, int part) {
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT, int SV>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(StageArgs A) {
     constexpr int NP = parts_of<TX, TZ>();
-    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ, SV>()];
+    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ, SV, MODE>()];
     const int slot = tile_of_block(blockIdx.x, A.ntiles * NP, A.xcd_remap);
     const int tile = slot / NP, part = slot % NP;
     if (A.flags && A.flags[tile] != A.want) return; // whole workgroup leaves before any barrier
@@ -888,7 +900,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel(St
 template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool KO_EXACT, int SV>
 __global__ __launch_bounds__(TX* TY* TZ, GR_FP64_MIN_WAVES) void stage_kernel_scan(StageArgs A) {
     constexpr int NT = TX * TY * TZ;
-    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ, SV>()];
+    __shared__ double lds[fp64_lds_doubles<TX, TY, TZ, SV, MODE>()];
     __shared__ int list[NT];
     __shared__ int nlist;
     const int per = (int)gridDim.x * NT;
