@@ -2,6 +2,7 @@
 launch per RK3 stage. Each block must come out exactly as when it is evolved alone
 (bitwise: the same kernels compute the same cells), and small uneven blocks agree with
 the oracle to the FP64 tolerance."""
+import os
 import sys
 
 import numpy as np
@@ -129,16 +130,31 @@ def c4_levels(ga, engine):
     return out
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_c4_layout_batched_equals_blocks_alone(engine, precision):
+@pytest.mark.parametrize("precision,variant", [(0, 0), (1, 0), (0, 2), (1, 2)])
+def test_c4_layout_batched_equals_blocks_alone(engine, precision, variant, tmp_path):
     """The batched-AMR workload itself (granite_amd.workloads.uneven_amr_levels: 250 blocks,
     edges from {16, 24, 32, 48, 64}, 4 levels): one RK3 step of every level in one stage launch
     per level; every block (FP64) or every fourth block (mixed) bit-identical to the block
-    evolved alone."""
+    evolved alone. With the FP64 halo staging through registers (variant 0, the default) and
+    straight to LDS in 8x4x8 parts (variant 2, taken from a tuning table for every block-size
+    class, so the batched levels and the single blocks alike run it)."""
     import granite_amd as ga
 
     from granite_amd import workloads
 
+    ga.tune_clear()
+    if variant:
+        table = os.path.join(str(tmp_path), "tune.txt")
+        with open(table, "w") as f:
+            f.write("".join(f"0 {c} 0 {variant}\n" for c in range(4)))
+        ga.tune_load(table)
+    try:
+        _c4_batched_equals_alone(ga, workloads, engine, precision)
+    finally:
+        ga.tune_clear()
+
+
+def _c4_batched_equals_alone(ga, workloads, engine, precision):
     p = ga.CCZ4Params()
     masses, pos = workloads.PUNCTURES
     levels = c4_levels(ga, engine)

@@ -134,7 +134,8 @@ def _fp64_tiles(ga, chi, n, threshold=0.2):
     return mask
 
 
-def test_mixed_form_routes_nonfinite_tiles_to_fp64(engine, oracle):
+@pytest.mark.parametrize("variant", [0, 2])
+def test_mixed_form_routes_nonfinite_tiles_to_fp64(engine, oracle, variant):
     """The mixed form classifies tiles whose smallest chi is below the threshold or not
     finite (the NaN and infinite chi cells, the 5e-5 cube) as FP64 tiles and runs the rest in
     FP32. (a) The RHS of the seeded state: the same non-finite set in both forms; on the tiles
@@ -143,7 +144,9 @@ def test_mixed_form_routes_nonfinite_tiles_to_fp64(engine, oracle):
     step: every non-finite value of the FP64 form is non-finite in the mixed form, which
     overflows in addition only where the FP64 state has grown beyond the FP32 algebra's range
     (|value| > 1.8e19, whose square overflows: K = 5.9e32 next to the chi = 5e-5 cube after the
-    step), and the repaired values are the same constants in both forms."""
+    step), and the repaired values are the same constants in both forms. Both forms run the
+    FP64 stage kernel with the given halo staging (0: through registers; 2: straight to LDS),
+    which in the mixed form covers the FP64 tiles from a small grid on a second stream."""
     import granite_amd as ga
 
     g, u = seeded_state(oracle)
@@ -152,8 +155,9 @@ def test_mixed_form_routes_nonfinite_tiles_to_fp64(engine, oracle):
     blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
     blk.array[...] = u
     lvl = ga.Level([blk], engine)
+    lvl.set_tuning(ga.TUNE_STAGE, 0, variant)
     lvl.upload()
-    say("seeded state, RHS in both precisions")
+    say(f"seeded state, RHS in both precisions, FP64 staging variant {variant}")
     r64 = lvl.rhs(0, p, ga.PREC_FP64)[I]
     r32 = lvl.rhs(0, p, ga.PREC_MIXED)[I]
     lvl.close()
@@ -176,8 +180,9 @@ def test_mixed_form_routes_nonfinite_tiles_to_fp64(engine, oracle):
         blk = ga.GridBlock(0, 0, g.n, g.lo, g.hi, g.ng)
         blk.array[...] = u
         lvl = ga.Level([blk], engine)
+        lvl.set_tuning(ga.TUNE_STAGE, 0, variant)
         lvl.upload()
-        say(f"seeded state, one RK3 step, precision {prec}")
+        say(f"seeded state, one RK3 step, precision {prec}, FP64 staging variant {variant}")
         lvl.rk3_step(p, 0.0625, precision=prec, bc_mode=ga.BC_SOMMERFELD, pre_fill=ga.BC_OUTFLOW)
         lvl.download()
         out[prec] = blk.array.copy()
