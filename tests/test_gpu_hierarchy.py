@@ -103,11 +103,27 @@ def test_transfers_bitwise(engine, oracle, case):
         L.close()
 
 
-@pytest.mark.parametrize("case,use_graph", [("merged", False), ("merged", True), ("split", True),
-                                            ("uncovered", True)])
-def test_coarse_steps_against_oracle(engine, oracle, case, use_graph):
+@pytest.mark.parametrize("case,use_graph,variant", [("merged", False, 0), ("merged", True, 0), ("split", True, 0),
+                                                    ("uncovered", True, 0), ("merged", True, 2)])
+def test_coarse_steps_against_oracle(engine, oracle, case, use_graph, variant, tmp_path):
+    """Two Berger-Oliger coarse steps of the device hierarchy against the oracle's, every level
+    within the FP64 bar; variant 2 takes the FP64 stage kernel's direct-to-LDS staging for
+    every level from a tuning table."""
     import granite_amd as ga
 
+    ga.tune_clear()
+    if variant:
+        table = os.path.join(str(tmp_path), "tune.txt")
+        with open(table, "w") as f:
+            f.write("".join(f"0 {c} 0 {variant}\n" for c in range(4)))
+        ga.tune_load(table)
+    try:
+        _coarse_steps_against_oracle(ga, engine, oracle, case, use_graph)
+    finally:
+        ga.tune_clear()
+
+
+def _coarse_steps_against_oracle(ga, engine, oracle, case, use_graph):
     geo, sph = sphere_geometry(case)
     masses = [0.5] * len(sph)
     pos = [list(s.center) for s in sph]
