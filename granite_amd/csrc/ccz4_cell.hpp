@@ -64,15 +64,16 @@ __host__ __device__ constexpr int sym_b(int p) { return p < 3 ? p : (p < 5 ? p -
 // Stencil constants of one block, computed on the host with the reference's operations:
 // ih = 1/(12 h) (ccz4.cpp:72,111), ih2 = 1/(12 h h) (ccz4.cpp:158), ihh = 1/(4 h_a h_b) for
 // (x,y),(x,z),(y,z) (ccz4.cpp:188), ko = sigma/(64 h) (ccz4.cpp:886-887).
-// kw[d][s] = ko[d] * (1,-6,15,-20,15,-6,1)[s], the per-tap products the reference forms
-// before accumulating (ccz4.cpp:921-924).
+// kw[d][s] = ko[d] * (1,-6,15,-20)[s], the per-tap products the reference forms before
+// accumulating (ccz4.cpp:921-924); taps 4..6 repeat taps 2..0 (the weights are symmetric, so
+// the reference's products are the same numbers), kept once to spare scalar registers.
 template <typename R>
 struct Stencil {
     R ih[3];
     R ih2[3];
     R ihh[3];
     R ko[3];
-    R kw[3][7];
+    R kw[3][4];
 };
 
 // Index strides of a staged array along x, y, z. The LDS tile uses compile-time strides
@@ -130,9 +131,9 @@ __host__ __device__ GR_INLINE R ko_axis_ordered(R acc, const R* kw, R m3, R m2, 
     acc = acc + kw[1] * m2;
     acc = acc + kw[2] * m1;
     acc = acc + kw[3] * c;
-    acc = acc + kw[4] * p1;
-    acc = acc + kw[5] * p2;
-    acc = acc + kw[6] * p3;
+    acc = acc + kw[2] * p1;
+    acc = acc + kw[1] * p2;
+    acc = acc + kw[0] * p3;
     return acc;
 }
 

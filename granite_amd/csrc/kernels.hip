@@ -1785,12 +1785,12 @@ inline unsigned grid_for(long n, int threads) { return (unsigned)((n + threads -
 
 Stencil<double> make_stencil(const BlockDesc& b, double ko_sigma) {
     Stencil<double> K;
-    static const double wts[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+    static const double wts[4] = {1.0, -6.0, 15.0, -20.0};
     for (int d = 0; d < 3; ++d) {
         K.ih[d] = 1.0 / (12.0 * b.dx[d]);
         K.ih2[d] = 1.0 / (12.0 * b.dx[d] * b.dx[d]);
         K.ko[d] = ko_sigma / (64.0 * b.dx[d]);
-        for (int s = 0; s < 7; ++s) K.kw[d][s] = K.ko[d] * wts[s];
+        for (int s = 0; s < 4; ++s) K.kw[d][s] = K.ko[d] * wts[s];
     }
     K.ihh[0] = 1.0 / (4.0 * b.dx[0] * b.dx[1]);
     K.ihh[1] = 1.0 / (4.0 * b.dx[0] * b.dx[2]);
@@ -1831,7 +1831,7 @@ static StageArgs stage_args(const Batch& B, const int sel[3], const CellParams& 
         A.K32.ih2[d] = (float)A.K.ih2[d];
         A.K32.ihh[d] = (float)A.K.ihh[d];
         A.K32.ko[d] = (float)A.K.ko[d];
-        for (int t = 0; t < 7; ++t) A.K32.kw[d][t] = (float)A.K.kw[d][t];
+        for (int t = 0; t < 4; ++t) A.K32.kw[d][t] = (float)A.K.kw[d][t];
     }
     A.P = P;
     A.cf = cf;

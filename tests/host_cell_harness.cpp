@@ -41,8 +41,8 @@ static void run(const int* n, int ng, const double* dx, const double* prm, const
         K.ih[d] = 1.0 / (12.0 * dx[d]);
         K.ih2[d] = 1.0 / (12.0 * dx[d] * dx[d]);
         K.ko[d] = prm[3] / (64.0 * dx[d]);
-        const double wts[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
-        for (int s = 0; s < 7; ++s) K.kw[d][s] = K.ko[d] * wts[s];
+        const double wts[4] = {1.0, -6.0, 15.0, -20.0};
+        for (int s = 0; s < 4; ++s) K.kw[d][s] = K.ko[d] * wts[s];
     }
     K.ihh[0] = 1.0 / (4.0 * dx[0] * dx[1]);
     K.ihh[1] = 1.0 / (4.0 * dx[0] * dx[2]);
