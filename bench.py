@@ -344,7 +344,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the level would shard "
+        print("bench.py measures one GPU only: the engine has no multi-GPU path (the level would shard "
               "per block across GPUs; see the sharding design). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
