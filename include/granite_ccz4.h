@@ -354,7 +354,8 @@ int gr_hierarchy_get_tuning(gr_hierarchy* h, int level, int* groups);
 int gr_hierarchy_set_tuning(gr_hierarchy* h, int level, int groups);
 /* The process-wide tuning table: write to / read from a text file (one line per entry:
  * kind class precision value), or empty it. Levels and hierarchies created afterwards use
- * the entries of their class; absent entries keep the defaults (staging 0, 4 groups). */
+ * the entries of their class; absent entries keep the defaults (staging 2 for a group of
+ * blocks with even row lengths and at least 40^3 interior cells in all, else 0; 4 groups). */
 int gr_tune_save(const char* path);
 int gr_tune_load(const char* path);
 int gr_tune_clear(void);
