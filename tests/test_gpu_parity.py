@@ -217,6 +217,35 @@ def test_rk3_step_parity(engine, oracle, case):
     lvl.close()
 
 
+def test_c1_gauge_wave_to_t1(engine, oracle, oracle_release):
+    """C1 at its full size: the gauge wave (setGaugeWaveData, A = 0.1, L = 1) on 32^3 cells of
+    [0, 1)^3 with periodic ghosts, 128 RK3 steps of dt = dx / 4 to t = 1, graph-replayed,
+    against the oracle's 128 steps: every field within max(1e-12, 2 x the departure of the
+    reference's Release build from the strict oracle over the same 128 steps). The Release
+    build itself departs by up to 2.5e-11 (Theta) by t = 1: round-off, accumulated by a
+    long evolution whose K and Theta stay near zero."""
+    import granite_amd as ga
+
+    g = Geom((32, 32, 32), 4, (0, 0, 0), (1, 1, 1))
+    u = oracle.init_gauge_wave(g, 0.1, 1.0)
+    lvl, blk = make_level(engine, g, u)
+    p = ga.CCZ4Params()
+    dt = 1.0 / 128
+    say("C1 gauge wave: 128 steps on the device")
+    for _ in range(128):
+        lvl.rk3_step(p, dt, bc_mode=1, pre_fill=-1, use_graph=True)
+    lvl.download()
+    say("C1 gauge wave: 128 steps of the oracle, strict and Release")
+    ref, rel = u.copy(), u.copy()
+    for _ in range(128):
+        oracle.rk3_step(g, ref, dt, bc_mode=1)
+        oracle_release.rk3_step(g, rel, dt, bc_mode=1)
+    e = group_rel(blk.array, ref)
+    release = group_rel(rel, ref)
+    assert (e <= np.maximum(1e-12, 2.0 * release)).all(), (e, release)
+    lvl.close()
+
+
 def test_graph_replay_equals_eager_and_is_deterministic(engine, oracle):
     import granite_amd as ga
 
