@@ -54,6 +54,14 @@
 // (55 KB for two buffers of an 8x4x4 part: two workgroups per CU); 1 = unpadded rows,
 // planes padded, lanes interleaved over y and z (38 KB: room for four workgroups per CU,
 // i.e. two waves per SIMD when the registers allow it)
+// field buffers of the direct-to-LDS staging, at most (2 or 3; fewer where more would not
+// leave two waves per SIMD room in the CU's LDS); the last stage's own limit
+#ifndef GR_DMA_BUFFERS
+#define GR_DMA_BUFFERS 3
+#endif
+#ifndef GR_DMA_BUFFERS_FINAL
+#define GR_DMA_BUFFERS_FINAL 2
+#endif
 #ifndef GR_MIXED_PREFETCH
 #define GR_MIXED_PREFETCH 1
 #endif
@@ -406,9 +414,12 @@ struct TileIO {
 // in 1 KiB chunks: lane l of a wave's load of chunk c writes doubles 128 c + 2 l, + 1 of the
 // buffer, fetched from the pair of x-neighbours the layout puts there (pad slots and slots
 // beyond the block fetch the field's first pair and are never read by an active cell).
-// The waves take the chunks in turn. A load's LDS data is complete once the wave's vector
-// memory counter drains, which the barrier that starts the next field waits for, so the
-// buffer of field p + 1 is loaded while field p is evaluated from the other one.
+// The waves take the chunks in turn. With NBUF field buffers, field p + NBUF - 1 is loaded
+// while field p is evaluated. A load's LDS data is complete once the wave's vector memory
+// counter has counted it back; the counter drains in issue order (loads and stores alike),
+// so the barrier that starts field p waits until no more of this wave's operations are
+// outstanding than the chunk loads it has issued since those of field p: the loads of the
+// fields after p stay in flight across it.
 // Needs 16-byte aligned pairs: an even row length (nt0) and 16-byte aligned buffers.
 template <int TX, int TY, int TZ, int MODE>
 struct TileIODma {
@@ -420,12 +431,21 @@ struct TileIODma {
     static constexpr int NW = T::NT / 64; // waves
     static constexpr int KW = (NCH + NW - 1) / NW; // chunk loads per wave
     static_assert(T::NT % 64 == 0, "whole waves");
-    // two field buffers; in the last stage also the new metric of every cell, from its
-    // combine to the algebraic constraints at the end, where that still lets two waves per
-    // SIMD (512 / NT workgroups) share the CU's 160 KB; otherwise the metric is stored at
-    // once and read back at the end
-    static constexpr bool METRIC_LDS = MODE == MODE_FINAL && (2 * LBUF + 6 * T::NT) * 8 * (512 / T::NT) <= 163840;
-    static constexpr int LDS_DOUBLES = 2 * LBUF + (METRIC_LDS ? 6 * T::NT : 0);
+    // field buffers: as many as the limit allows while two waves per SIMD (512 / NT
+    // workgroups) still share the CU's 160 KB
+    static constexpr int WG_PER_CU = 512 / T::NT;
+    static constexpr int NBUF_MAX = MODE == MODE_FINAL ? GR_DMA_BUFFERS_FINAL : GR_DMA_BUFFERS;
+    static constexpr int NBUF = (NBUF_MAX >= 3 && 3 * LBUF * 8 * WG_PER_CU <= 163840) ? 3 : 2;
+    // in the last stage also the new metric of every cell, from its combine to the
+    // algebraic constraints at the end, where it still fits beside the buffers; otherwise
+    // the metric is stored at once and read back at the end
+    static constexpr bool METRIC_LDS = MODE == MODE_FINAL && (NBUF * LBUF + 6 * T::NT) * 8 * WG_PER_CU <= 163840;
+    static constexpr int LDS_DOUBLES = NBUF * LBUF + (METRIC_LDS ? 6 * T::NT : 0);
+    // vector memory operations that every wave issues after a field's chunk loads and
+    // before the barrier of that field: the unconditional chunk loads of the next field
+    // (the last chunk of a wave may be skipped)
+    static constexpr int YOUNGER = NBUF == 3 ? KW - 1 : 0;
+    static_assert(YOUNGER < 16, "vmcnt field of s_waitcnt");
 
     double* lds; // LDS_DOUBLES doubles
     const gdouble* in;
@@ -441,12 +461,12 @@ struct TileIODma {
     StageCoef cf;
     unsigned doff[KW]; // byte offset of this lane's pair in each of its wave's chunks
     int tid;
-    double unb;
     double unv[NV];
     double hA[6];
-    // the barrier that starts a field waits for every outstanding vector memory operation
-    // (it must, for the direct LDS loads), so a field's result is stored after the next
-    // barrier instead of just before it: the store then has a whole field to complete
+    // the barrier that starts a field waits for every vector memory operation issued before
+    // the chunk loads of the field after it, stores included, so a field's result is stored
+    // after the next barrier instead of just before it: the store then has a whole field to
+    // complete
     double pend_val;
     int pend_v;
 
@@ -493,7 +513,8 @@ struct TileIODma {
             doff[k] = 8u * (unsigned)(ok && need ? gx + nt0 * (gy + nt1 * gz) : ox + nt0 * (oy + nt1 * oz));
         }
         pend_v = -1;
-        prefetch(0);
+#pragma unroll
+        for (int p = 0; p < NBUF - 1; ++p) prefetch(p);
     }
 
     __device__ __forceinline__ void flush() {
@@ -512,7 +533,7 @@ struct TileIODma {
 #ifdef __HIP_DEVICE_COMPILE__
         asm volatile("" : "+s"(L));
 #endif
-        L += (pos & 1) * LBUF;
+        L += (pos % NBUF) * LBUF;
 #pragma unroll
         for (int k = 0; k < KW; ++k) {
             const int c = wave + k * NW; // uniform over the wave
@@ -527,7 +548,7 @@ struct TileIODma {
                                                  (__attribute__((address_space(3))) void*)(L + c * CH), 16, 0, 0);
             }
         }
-        if (MODE != MODE_RHS) unb = gload(un + v * stride, cb);
+        if (MODE != MODE_RHS) unv[v] = gload(un + v * stride, cb);
     }
 
     // ---- Stager
@@ -539,11 +560,27 @@ struct TileIODma {
     __device__ FixedStrides<1, T::BX, T::PLANE> strides() const { return {}; }
     __device__ const double* field(int v) {
         const int pos = field_pos(v);
-        __syncthreads(); // field pos has landed in its buffer; every thread is done with pos - 1
+        if (NBUF == 2) {
+            __syncthreads(); // field pos has landed in its buffer; every thread is done with pos - 1
+        } else {
+            // this wave's loads of field pos have landed (only those of pos + 1 may still be
+            // outstanding: none after the last field) and its LDS accesses are done; then
+            // the barrier: every wave's loads of pos have landed and every thread is done
+            // with field pos - 1, whose buffer the prefetch below refills. The barrier is
+            // written as a statement that may touch any memory, so the compiler keeps every
+            // access on its side of it; a workgroup fence would also drain the loads of
+            // pos + 1 (they write LDS)
+            if (pos + 1 < NV)
+                __builtin_amdgcn_s_waitcnt(0x0070 | YOUNGER); // vmcnt(YOUNGER) lgkmcnt(0)
+            else
+                __builtin_amdgcn_s_waitcnt(0x0070);
+#ifdef __HIP_DEVICE_COMPILE__
+            asm volatile("s_barrier" ::: "memory");
+#endif
+        }
         flush();
-        if (MODE != MODE_RHS) unv[v] = unb;
-        prefetch(pos + 1);
-        return lds + (pos & 1) * LBUF + cl;
+        prefetch(pos + NBUF - 1);
+        return lds + (pos % NBUF) * LBUF + cl;
     }
 
     // ---- Sink
@@ -562,7 +599,7 @@ struct TileIODma {
         // read back at the end) for the algebraic constraints, instead of in registers across
         // the rest of the RHS; A~ comes last and is held in registers
         if (METRIC_LDS && v >= GXX && v < GXX + 6) {
-            lds[2 * LBUF + (v - GXX) * T::NT + tid] = val;
+            lds[NBUF * LBUF + (v - GXX) * T::NT + tid] = val;
             return;
         }
         if (MODE == MODE_FINAL && v >= AXX && v < AXX + 6) {
@@ -578,7 +615,7 @@ struct TileIODma {
         if (MODE != MODE_FINAL) return;
         double hg[6];
 #pragma unroll
-        for (int a = 0; a < 6; ++a) hg[a] = METRIC_LDS ? lds[2 * LBUF + a * T::NT + tid] : gload(out + (GXX + a) * stride, cb);
+        for (int a = 0; a < 6; ++a) hg[a] = METRIC_LDS ? lds[NBUF * LBUF + a * T::NT + tid] : gload(out + (GXX + a) * stride, cb);
         algebraic_exact(hg, hA);
         if (!active) return;
 #pragma unroll
