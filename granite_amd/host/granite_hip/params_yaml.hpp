@@ -258,7 +258,7 @@ inline YNode parse_yaml(const std::string& text) {
         int ind = 0;
         while (ind < (int)s.size() && s[ind] == ' ') ++ind;
         if (ind < (int)s.size() && s[ind] == '\t') throw std::runtime_error("yaml line " + std::to_string(number) + ": tab indentation");
-        yaml_detail::Line line{ind, yaml_detail::trim(s), number};
+        yaml_detail::Line line{ind, yaml_detail::trim(s), number, false, std::string()};
         char chomp = ' ';
         const char style = yaml_detail::is_item_text(line.text) ? '\0' : yaml_detail::block_header(line.text, &chomp);
         if (style) {
