@@ -29,6 +29,7 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector
 N = 128
 HALF = 16.0
+N_NOISY = 192
 DT = 0.25 * (2 * HALF / N)
 BYTES_PER_CELL_STEP = 1408  # 3 stages x (22-field read + write) + 2 x 22-field read of u^n, FP64
 
@@ -83,6 +84,15 @@ def cpu_baseline():
     if one.returncode == 0:
         r1 = json.loads(one.stdout.strip().splitlines()[-1])
         res["single_thread"] = {"value": r1["value"], "cores": r1["cores"], "sample": r1["sample"]}
+    # the 192^3 block with 1e-8 noise on the metric and K (the noisy_192_mixed workload), same
+    # cores: the median of 3 runs of one step's RHS work
+    big = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--n", str(N_NOISY), "--half-width",
+                          str(HALF), "--steps", "1", "--noise", "1e-8"], capture_output=True, text=True, env=env,
+                         timeout=900, cwd=ROOT)
+    if big.returncode == 0:
+        r3 = json.loads(big.stdout.strip().splitlines()[-1])
+        res["noisy_192"] = {"value": r3["value"], "cores": r3["cores"], "sample": r3["sample"],
+                            "seconds_per_step": r3["seconds_per_step"]}
     return res
 
 
@@ -200,7 +210,7 @@ def run_noisy_192(ga, eng, p, steps, tune=True, autotune=True):
     the oracle to 1e-12), then `steps` RK3 steps of each form with their ||H||_2."""
     import numpy as np
 
-    n, half = 192, 16.0
+    n, half = N_NOISY, HALF
     dt = 0.25 * (2 * half / n)
     noise_fields = [1, 2, 3, 4, 5, 6, 13]
     out = {"block": [n] * 3, "dt": dt, "noise": 1e-8, "steps": steps}
@@ -509,6 +519,8 @@ def main():
     }
     if cpu.get("value"):
         line["gpu_over_cpu"] = value / cpu["value"]
+    if noisy and (cpu.get("noisy_192") or {}).get("value"):
+        noisy["gpu_over_cpu"] = noisy["fp64"]["value"] / cpu["noisy_192"]["value"]
     if args.tune_save:
         ga.tune_save(args.tune_save)
     print(json.dumps(line), flush=True)

@@ -34,12 +34,19 @@ def main():
     ap.add_argument("--half-width", type=float, default=16.0)
     ap.add_argument("--steps", type=int, default=5, help="RK3 steps' worth of RHS work per timed run")
     ap.add_argument("--runs", type=int, default=3, help="timed runs; the median is reported")
+    ap.add_argument("--noise", type=float, default=0.0,
+                    help="amplitude of the noise added to the conformal metric and K (0: none)")
+    ap.add_argument("--seed", type=int, default=20240611)
     args = ap.parse_args()
     n = args.n
     w = args.half_width
     g = Geom((n, n, n), 4, (-w,) * 3, (w,) * 3)
     port = Oracle(fast=True)
     u = port.init_bl(g, [1.0], [[0.0, 0.0, 0.0]])
+    block = f"{n}^3 Brill-Lindquist block"
+    if args.noise > 0:
+        port.add_noise(g, u, args.seed, args.noise, [1, 2, 3, 4, 5, 6, 13])
+        block += f" with {args.noise:g} noise on the conformal metric and K"
     threads = port.num_threads()
     times = []
     if Reference.available(fast=True):
@@ -53,7 +60,7 @@ def main():
             times.append((time.perf_counter() - t0) / args.steps)
         kind = "reference"
         sample = (f"median of {args.runs} runs, each {args.steps} RK3 steps' RHS work (3 x computeRHS + "
-                  f"applyDissipation per step) of the compiled reference on the {n}^3 Brill-Lindquist block, after "
+                  f"applyDissipation per step) of the compiled reference on the {block}, after "
                   f"one warm-up evaluation; the combine / floor / ghost passes of a step are excluded")
     else:
         stage = g.zeros()
@@ -65,8 +72,7 @@ def main():
                 port.rk3_step(g, u, 0.0625, 0, stage=stage, rhs=rhs)
             times.append((time.perf_counter() - t0) / args.steps)
         kind = "port"
-        sample = (f"median of {args.runs} runs of {args.steps} full SSP-RK3 steps of the oracle on the {n}^3 "
-                  f"Brill-Lindquist block")
+        sample = f"median of {args.runs} runs of {args.steps} full SSP-RK3 steps of the oracle on the {block}"
     dt = sorted(times)[len(times) // 2]
     cells = n ** 3
     print(json.dumps({"value": cells / dt, "unit": "cell-updates/s", "cores": threads, "kind": kind,
