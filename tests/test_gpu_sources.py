@@ -36,7 +36,10 @@ def bl_block(oracle, n=(20, 16, 12), lo=(-2.5, -2.0, -1.5), hi=(2.5, 2.0, 1.5)):
     return g, oracle.init_bl(g, [1.0], [[0.05, -0.02, 0.01]])
 
 
-def test_rk3_step_with_sources(engine, oracle):
+@pytest.mark.parametrize("variant", [0, 2])
+def test_rk3_step_with_sources(engine, oracle, variant):
+    """Both halo stagings of the FP64 stage kernel (registers; direct loads into LDS, whose
+    field barriers count the source loads among the operations in flight)."""
     import granite_amd as ga
 
     g, u = bl_block(oracle)
@@ -46,10 +49,11 @@ def test_rk3_step_with_sources(engine, oracle):
     lvl = ga.Level([blk], engine)
     lvl.upload()
     lvl.set_sources(0, *[t.data_ptr() for t in dev])
+    lvl.set_tuning(ga.TUNE_STAGE, 0, variant)
     p = ga.CCZ4Params()
     ref = u.copy()
     for step in range(2):
-        say(f"rk3 with sources, step {step}")
+        say(f"rk3 with sources, staging variant {variant}, step {step}")
         lvl.rk3_step(p, 0.0625, pre_fill=ga.BC_OUTFLOW, use_graph=step == 1)
         oracle.fill_ghosts(g, ref, 2)
         oracle.rk3_step(g, ref, 0.0625, 0, sources=host)
