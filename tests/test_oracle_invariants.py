@@ -109,6 +109,55 @@ def test_rk3_linear_shift_decay_exact(oracle):
     assert np.array_equal(u[V["CHI"]], np.ones_like(u[0]))
 
 
+def _stage_sequence(oracle, g, u0, dt, floors_each_stage=True, project_each_stage=False):
+    """One SSP-RK3 step composed from separately pinned pieces in the order of
+    TimeIntegrator::sspRK3Step (src/main.cpp:673-689): per stage the RHS (orc_rhs, bitwise
+    equal to the compiled reference), the combine d = a1 s1 + a2 s2 + a3 dt r on interior
+    cells (src/main.cpp:306-335, written out here in numpy), the floors on every cell
+    (src/main.cpp:588-608), after the last stage only the algebraic constraints
+    (src/main.cpp:612-671), then the 1/r ghost fill (src/main.cpp:351-521). The keyword
+    switches build the misplaced orders the check must reject."""
+    I = g.interior()
+    u = u0.copy()
+    stage = g.zeros()
+    coef = [(1.0, "u", 0.0, "u", 1.0), (0.75, "u", 0.25, "s", 0.25), (1.0 / 3.0, "u", 2.0 / 3.0, "s", 2.0 / 3.0)]
+    for k, (a1, s1, a2, s2, a3) in enumerate(coef):
+        src = u if k == 0 else stage
+        r = oracle.rhs(g, src)
+        dst = u if k == 2 else stage
+        x1 = (u if s1 == "u" else stage)[I].copy()
+        x2 = (u if s2 == "u" else stage)[I].copy()
+        dst[I] = a1 * x1 + a2 * x2 + a3 * dt * r[I]
+        if floors_each_stage or k == 2:
+            oracle.floors(g, dst)
+        if project_each_stage or k == 2:
+            oracle.algebraic(g, dst)
+        oracle.fill_ghosts(g, dst, 0)
+    return u
+
+
+def test_rk3_step_is_the_reference_stage_sequence(oracle):
+    """orc_rk3_step equals, bit for bit, the reference's stage sequence composed from pinned
+    pieces; on a state with chi above its cap and below its floor, a conformal metric off unit
+    determinant and a traced A~, floors applied only at the end or the algebraic projection
+    applied at every stage give a different step."""
+    g = Geom((12, 10, 8), 4, (-1.5, -1.25, -1.0), (1.5, 1.25, 1.0))
+    u0 = oracle.add_noise(g, oracle.init_bl(g, [1.0], [[0.05, -0.02, 0.01]]), 5, 1e-3, range(22))
+    u0[V["GXX"]:V["GXX"] + 6, 5:9, 5:10, 5:11] *= 1.2
+    for a in (0, 3, 5):  # xx, yy, zz
+        u0[V["AXX"] + a, 4:10, 4:11, 4:12] += 0.01
+    u0[V["CHI"], 6, 6, 6:9] = 2.0  # above the 1.5 cap through every stage
+    u0[V["CHI"], 8, 9, 6] = 5e-5
+    dt = 0.05
+    step = u0.copy()
+    oracle.rk3_step(g, step, dt, bc_mode=0)
+    assert np.isfinite(step).all()
+    assert np.array_equal(step.view(np.uint64), _stage_sequence(oracle, g, u0, dt).view(np.uint64))
+    for kw in (dict(floors_each_stage=False), dict(project_each_stage=True)):
+        other = _stage_sequence(oracle, g, u0, dt, **kw)
+        assert normwise_rel(other[g.interior()], step[g.interior()]).max() > 1e-10, kw
+
+
 def test_rk3_with_uniform_sources_follows_the_reduced_ode(oracle):
     """Spatially uniform flat data with uniform matter sources rho, S (S_i = S_ij = 0), zero
     shift, periodic ghosts: every derivative and the Ricci tensor vanish, the metric and A~
