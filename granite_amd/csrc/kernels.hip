@@ -320,6 +320,9 @@ struct TileIO {
     double buf[PF][NL];
     double unb[PF];
     double unv[NV];
+    // u^n is the stage input itself (the first stage): the u^n value of a cell is then the
+    // centre value the combine already receives, and is not loaded again (uniform)
+    bool un_in;
     double hg[6], hA[6]; // last stage: new metric and A~ held for the algebraic constraints
 
     __device__ void init(int tid, int ox, int oy, int oz, int nt0, int nt1, int nt2) {
@@ -346,7 +349,7 @@ struct TileIO {
         const gdouble* f = in + v * stride;
 #pragma unroll
         for (int i = 0; i < NL; ++i) buf[slot][i] = gload_sv(f, goff[i]);
-        if (MODE != MODE_RHS) unb[slot] = gload(un + v * stride, cb);
+        if (MODE != MODE_RHS && !un_in) unb[slot] = gload(un + v * stride, cb);
     }
 
     // writes the prefetched field at position pos into its LDS buffer
@@ -356,7 +359,7 @@ struct TileIO {
         double* L = lds + (pos & 1) * LBUF;
 #pragma unroll
         for (int i = 0; i < NL; ++i) L[loff[i]] = buf[slot][i];
-        if (MODE != MODE_RHS) unv[FIELD_ORDER[pos]] = unb[slot];
+        if (MODE != MODE_RHS && !un_in) unv[FIELD_ORDER[pos]] = unb[slot];
     }
 
     // ---- Stager
@@ -381,7 +384,7 @@ struct TileIO {
             if (active) gstore(out + v * stride, cb, rr);
             return;
         }
-        double val = combine_exact(cf.a1, unv[v], cf.a2, uin, cf.a3dt, rr);
+        double val = combine_exact(cf.a1, un_in ? uin : unv[v], cf.a2, uin, cf.a3dt, rr);
         if (v == CHI) val = floor_chi(val);
         if (v == ALPHA) val = floor_alpha(val);
         // (one wave per SIMD: the held metric costs no occupancy here, and reading it back at
@@ -462,6 +465,7 @@ struct TileIODma {
     unsigned doff[KW]; // byte offset of this lane's pair in each of its wave's chunks
     int tid;
     double unv[NV];
+    bool un_in; // u^n is the stage input (the first stage): not loaded again, as in TileIO
     double hA[6];
     // the barrier that starts a field waits for every vector memory operation issued before
     // the chunk loads of the field after it, stores included, so a field's result is stored
@@ -548,7 +552,7 @@ struct TileIODma {
                                                  (__attribute__((address_space(3))) void*)(L + c * CH), 16, 0, 0);
             }
         }
-        if (MODE != MODE_RHS) unv[v] = gload(un + v * stride, cb);
+        if (MODE != MODE_RHS && !un_in) unv[v] = gload(un + v * stride, cb);
     }
 
     // ---- Stager
@@ -592,7 +596,7 @@ struct TileIODma {
             pend_val = rr;
             return;
         }
-        double val = combine_exact(cf.a1, unv[v], cf.a2, uin, cf.a3dt, rr);
+        double val = combine_exact(cf.a1, un_in ? uin : unv[v], cf.a2, uin, cf.a3dt, rr);
         if (v == CHI) val = floor_chi(val);
         if (v == ALPHA) val = floor_alpha(val);
         // last stage: the new metric waits in LDS (this thread's own slots; or in the output,
@@ -886,6 +890,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& A, const TileRef& R,
     io.lds = lds;
     io.in = R.in;
     io.un = R.un;
+    io.un_in = R.un == R.in;
     io.out = R.out;
     io.stride = b.stride;
     io.cidx = cidx;
