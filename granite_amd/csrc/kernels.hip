@@ -636,7 +636,9 @@ struct TileIODma {
 // tile (the field at the tile's first cell): every stencil's weights sum to zero, so
 // derivatives of the deltas are derivatives of the field, while small structure on top of
 // an O(1) field (1e-8 noise on the metric) keeps FP32 relative precision of the delta.
-template <int TX, int TY, int TZ, int MODE, int PF>
+// UN_IN: u^n is the stage input itself (the first stage, chosen at launch): its value at the
+// cell is the stage input's centre value, so it is not loaded separately
+template <int TX, int TY, int TZ, int MODE, int PF, bool UN_IN = false>
 struct TileIO32 {
     using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0, 4>;
     static constexpr int NL = (T::TOTAL + T::NT - 1) / T::NT;
@@ -687,7 +689,7 @@ struct TileIO32 {
         for (int i = 0; i < NL; ++i) buf[slot][i] = gload(f, goff[i]);
         refb[slot] = f[ridx];
         inb[slot] = gload(f, cb);
-        if (MODE != MODE_RHS) unb[slot] = gload(un + v * stride, cb);
+        if (MODE != MODE_RHS && !UN_IN) unb[slot] = gload(un + v * stride, cb);
     }
 
     __device__ __forceinline__ void stage_to_lds(int pos) {
@@ -699,7 +701,7 @@ struct TileIO32 {
         for (int i = 0; i < NL; ++i) L[loff[i]] = (float)(buf[slot][i] - r);
         const int v = FIELD_ORDER[pos];
         inv[v] = inb[slot];
-        if (MODE != MODE_RHS) unv[v] = unb[slot];
+        if (MODE != MODE_RHS) unv[v] = UN_IN ? inb[slot] : unb[slot];
     }
 
     // ---- Stager
@@ -988,7 +990,7 @@ constexpr int FP64_SCAN_GRID = 256 * 256 / (FP64_TX * FP64_TY * FP64_TZ);
 
 // one part of TX x TY x TZ cells (TILE_X / TX parts along x) of a launch tile per workgroup;
 // the tile is the unit of the FP32 / FP64 choice
-template <int TX, int TY, int TZ, int MODE, bool HAS_SRC>
+template <int TX, int TY, int TZ, int MODE, bool HAS_SRC, bool UN_IN = false>
 __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_mixed(StageArgs A) {
     using T = TileGeom<TX, TY, TZ, GR_LDS_PAD32 != 0, 4>;
     constexpr int NP = GR_TILE_X / TX;
@@ -1009,7 +1011,7 @@ __global__ __launch_bounds__(TX* TY* TZ, GR_MIXED_MIN_WAVES) void stage_kernel_m
     const int gi = b.ng + (active ? ix : 0), gj = b.ng + (active ? iy : 0), gk = b.ng + (active ? iz : 0);
     const long cidx = gi + (long)b.nt[0] * (gj + (long)b.nt[1] * gk);
 
-    TileIO32<TX, TY, TZ, MODE, GR_MIXED_PREFETCH> io;
+    TileIO32<TX, TY, TZ, MODE, GR_MIXED_PREFETCH, UN_IN> io;
     io.lds = lds;
     io.in = R.in;
     io.un = R.un;
@@ -1966,8 +1968,14 @@ hipError_t launch_stage_fp64(const Batch& B, const int sel[3], bool has_src, con
 template <int MODE, bool HAS_SRC>
 static hipError_t launch_mixed_t(const StageArgs& A, hipStream_t s) {
     constexpr int PX = TILE_X / GR_MIXED_SPLIT;
-    stage_kernel_mixed<PX, TILE_Y, TILE_Z, MODE, HAS_SRC>
-        <<<dim3(A.ntiles * GR_MIXED_SPLIT), dim3(PX * TILE_Y * TILE_Z), 0, s>>>(A);
+    // the first stage (u^n is the stage input) has an instance of its own, so that no load
+    // of the kernel's register prefetch is conditional
+    if (MODE == MODE_STAGE && A.sel_un == A.sel_in)
+        stage_kernel_mixed<PX, TILE_Y, TILE_Z, MODE, HAS_SRC, MODE == MODE_STAGE>
+            <<<dim3(A.ntiles * GR_MIXED_SPLIT), dim3(PX * TILE_Y * TILE_Z), 0, s>>>(A);
+    else
+        stage_kernel_mixed<PX, TILE_Y, TILE_Z, MODE, HAS_SRC>
+            <<<dim3(A.ntiles * GR_MIXED_SPLIT), dim3(PX * TILE_Y * TILE_Z), 0, s>>>(A);
     return hipGetLastError();
 }
 
