@@ -347,6 +347,30 @@ def test_bad_arguments_raise(engine):
     assert lib.gr_level_create(engine.handle, 0, None, C.byref(out)) == -1
 
 
+def test_full_size_step_against_oracle(engine, oracle):
+    """The headline workload itself: the 128^3 Brill-Lindquist block (M = 1 at the origin, dx =
+    0.25 on [-16,16]^3), one graph-replayed RK3 step with the level-0 outflow fill before it
+    and 1/r ghosts after every stage, untuned (the direct-to-LDS staging a level of this size
+    starts on), against the oracle's step: every field within 1e-12 of its tensor group. The
+    oracle's step takes about half a minute on one core."""
+    import granite_amd as ga
+
+    n = 128
+    g = Geom((n, n, n), 4, (-16.0,) * 3, (16.0,) * 3)
+    u = oracle.init_bl(g, [1.0], [[0.0, 0.0, 0.0]])
+    lvl, blk = make_level(engine, g, u)
+    assert lvl.tuning(ga.TUNE_STAGE, 0) == 2
+    say("full size 128^3: one step on the device")
+    lvl.rk3_step(ga.CCZ4Params(), 0.0625, pre_fill=ga.BC_OUTFLOW, use_graph=True)
+    lvl.download()
+    lvl.close()
+    say("full size 128^3: one step of the oracle")
+    oracle.fill_ghosts(g, u, 2)
+    oracle.rk3_step(g, u, 0.0625, 0)
+    e = group_rel(blk.array, u)
+    assert e.max() < TOL, e
+
+
 def test_full_size_block_properties(engine):
     """The 128^3 Brill-Lindquist block (dx = 0.25 on [-16,16]^3): after two graph-replayed
     steps the state is finite and mirror-symmetric about x = 0 (scalars even, x-odd tensor
